@@ -1,0 +1,558 @@
+/*
+ * ct_oracle.c -- CPU restatement of the reference's 2-D CT hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker.  The product path (ct_recon_extensions_amd) never
+ * links, loads or falls back to it.
+ *
+ * Every function restates one routine of the reference's single-threaded CPU
+ * implementation (the src/ directory of ltldlzltl/CT_RECON_EXTENSIONS), keeping its
+ * arithmetic type, its evaluation order and its edge rules, so that the
+ * results agree with the compiled reference to the last bit where the
+ * reference is well defined.  Where the reference is broken (the transposes
+ * of bp_par_2d and bp_fan_2d_angle, the unsigned-underflow edge of the
+ * ray-driven FP transpose) the oracle states the mathematically intended
+ * operator instead, and says so at the function.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   - against golden vectors written by the compiled reference
+ *     (oracle/_ref, tests/golden/make_golden.py): prep tables, fan BP,
+ *     fan BP transpose, fan-angle BP, par BP, ray-driven FP and its
+ *     transpose, ramp filter and its transpose, fan weighting;
+ *   - against closed forms / brute force: ramp taps (Ram-Lak), the filter
+ *     (numpy.convolve in FP64), constant-sinogram BP identities, the disk
+ *     chord length for FP, fan weights;
+ *   - transposes without a usable reference counterpart (par BP transpose,
+ *     fan-angle BP transpose) are pinned by the dot-product adjoint test
+ *     against their pinned forward operators, on sizes where a planted
+ *     index or sign error moves the inner product far beyond rounding.
+ *
+ * Build: oracle/Makefile -> oracle/libct_oracle.so (gcc -O2, C99, no FMA
+ * contraction, so the FP64 expressions round exactly like the reference's
+ * g++ -O3 -std=c++11 build on x86-64).
+ */
+#include <math.h>
+#include <stddef.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#ifndef M_PI_4
+#define M_PI_4 0.78539816339744830962
+#endif
+
+/* One geometry record for every operator (the union of the reference's
+ * Param structs, include/bp_fan_2d.h:20-59, include/bp_par_2d.h:20-55,
+ * include/fp_par_2d.h:20-55, include/filter.h:21-36,
+ * include/fan_weighting.h:21-39). */
+typedef struct {
+    unsigned int ns, na;
+    double ds, orbit, offset_s, orbit_start;
+    unsigned int nx, ny;
+    double dx, dy, offset_x, offset_y;
+    double dso, dsd, fov;
+    int type;            /* filter: 0 par, 1 fan, 2 flat; weighting: 1 fan, 2 flat */
+    unsigned int nrow;   /* filter / weighting rows */
+} orc_geom;
+
+/* ------------------------------------------------------------------ */
+/* Fan-beam tables: src/bp_fan_2d.cc:18-44 (GradPrep :96-122 identical). */
+int orc_fan_prep(const orc_geom* g, double* xpos, double* ypos, double* sincostbl)
+{
+    const double centx = ((double)(g->nx - 1)) / 2 + g->offset_x;
+    const double centy = ((double)(g->ny - 1)) / 2 + g->offset_y;
+    double angle = g->orbit_start;
+    double posx = -centx * g->dx, posy = centy * g->dy;
+    unsigned int i;
+    for (i = 0; i < g->na; ++i) {
+        sincostbl[2 * i] = sin(angle);
+        sincostbl[2 * i + 1] = cos(angle);
+        angle += g->orbit;
+    }
+    for (i = 0; i < g->nx; ++i) { xpos[i] = posx; posx += g->dx; }
+    for (i = 0; i < g->ny; ++i) { ypos[i] = posy; posy -= g->dy; }
+    return 1;
+}
+
+/* Fan-beam tables from caller angles: src/bp_fan_2d_angle.cc:18-45. */
+int orc_fan_angle_prep(const orc_geom* g, const double* angles, double* xpos,
+                       double* ypos, double* sincostbl)
+{
+    const double centx = ((double)(g->nx - 1)) / 2 + g->offset_x;
+    const double centy = ((double)(g->ny - 1)) / 2 + g->offset_y;
+    double posx = -centx * g->dx, posy = centy * g->dy;
+    unsigned int i;
+    for (i = 0; i < g->na; ++i) {
+        sincostbl[2 * i] = sin(angles[i]);
+        sincostbl[2 * i + 1] = cos(angles[i]);
+    }
+    for (i = 0; i < g->nx; ++i) { xpos[i] = posx; posx += g->dx; }
+    for (i = 0; i < g->ny; ++i) { ypos[i] = posy; posy -= g->dy; }
+    return 1;
+}
+
+/* Scale applied by bp_fan_2d: |orbit| / round(|na*orbit|/pi)
+ * (src/bp_fan_2d.cc:58,83).  bp_fan_2d_angle applies none (:83). */
+static double fan_scale(const orc_geom* g)
+{
+    const double factor = round(fabs(g->na * g->orbit) / M_PI);
+    return fabs(g->orbit) / factor;
+}
+
+/* Pixel-driven fan BP, src/bp_fan_2d.cc:46-91 (scaled=1) and
+ * src/bp_fan_2d_angle.cc:47-91 (scaled=0).  FP64 accumulation. */
+#define DEF_FAN_BP(T, SUF)                                                        \
+int orc_fan_bp_##SUF(const orc_geom* g, const T* proj, T* img, const double* xpos, \
+                     const double* ypos, const double* sc, int scaled)            \
+{                                                                                 \
+    const double cents = ((double)(g->ns - 1)) / 2 + g->offset_s;                 \
+    const double factor = round(fabs(g->na * g->orbit) / M_PI);                   \
+    unsigned int ix, iy, ia, is1, is2;                                            \
+    for (iy = 0; iy < g->ny; ++iy) {                                              \
+        for (ix = 0; ix < g->nx; ++ix) {                                          \
+            double sum = 0.0;                                                     \
+            const double x = xpos[ix], y = ypos[iy];                              \
+            for (ia = 0; ia < g->na; ++ia) {                                      \
+                const double sb = sc[2 * ia], cb = sc[2 * ia + 1];                \
+                const double d = g->dso + x * sb - y * cb;                        \
+                const double r = x * cb + y * sb;                                 \
+                const double gam = atan2(r, d);                                   \
+                const double s = gam * g->dsd / g->ds + cents;                    \
+                const double w = g->dsd * g->dsd / (d * d + r * r);               \
+                if (s >= 0 && s <= g->ns - 1) {                                   \
+                    double u;                                                     \
+                    is1 = (unsigned int)floor(s);                                 \
+                    is2 = (unsigned int)ceil(s);                                  \
+                    u = s - is1;                                                  \
+                    sum += ((1 - u) * proj[is1 + ia * g->ns]                      \
+                            + u * proj[is2 + ia * g->ns]) * w;                    \
+                }                                                                 \
+            }                                                                     \
+            img[ix + iy * g->nx] = scaled ? (T)(sum * fabs(g->orbit) / factor)    \
+                                          : (T)sum;                               \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_FAN_BP(float, f)
+DEF_FAN_BP(double, d)
+
+/* Transpose of the fan BP.  scaled=1 restates src/bp_fan_2d.cc:124-170
+ * (accumulation in T, scale applied at the end in T).  scaled=0 is the
+ * transpose of the unscaled bp_fan_2d_angle; the reference's own version
+ * (src/bp_fan_2d_angle.cc:124-165) divides by the weights and is not a
+ * transpose, so this branch is pinned by the adjoint test instead. */
+#define DEF_FAN_BPT(T, SUF)                                                       \
+int orc_fan_bpt_##SUF(const orc_geom* g, const T* img, T* grad, const double* xpos, \
+                      const double* ypos, const double* sc, int scaled)           \
+{                                                                                 \
+    const double cents = ((double)(g->ns - 1)) / 2 + g->offset_s;                 \
+    unsigned int ix, iy, ia, is1, is2;                                            \
+    size_t i, n = (size_t)g->ns * g->na;                                          \
+    for (i = 0; i < n; ++i) grad[i] = 0.0;                                        \
+    for (iy = 0; iy < g->ny; ++iy) {                                              \
+        for (ix = 0; ix < g->nx; ++ix) {                                          \
+            const T v = img[ix + iy * g->nx];                                     \
+            const double x = xpos[ix], y = ypos[iy];                              \
+            for (ia = 0; ia < g->na; ++ia) {                                      \
+                const double sb = sc[2 * ia], cb = sc[2 * ia + 1];                \
+                const double d = g->dso + x * sb - y * cb;                        \
+                const double r = x * cb + y * sb;                                 \
+                const double w = g->dsd * g->dsd / (d * d + r * r);               \
+                const double s = atan2(r, d) * g->dsd / g->ds + cents;            \
+                if (s >= 0 && s <= g->ns - 1) {                                   \
+                    double u;                                                     \
+                    is1 = (unsigned int)floor(s);                                 \
+                    is2 = (unsigned int)ceil(s);                                  \
+                    u = s - is1;                                                  \
+                    grad[is1 + ia * g->ns] += v * (T)(w * (1 - u));               \
+                    grad[is2 + ia * g->ns] += v * (T)(w * u);                     \
+                }                                                                 \
+            }                                                                     \
+        }                                                                         \
+    }                                                                             \
+    if (scaled) {                                                                 \
+        const T wt = (T)fan_scale(g);                                             \
+        for (i = 0; i < n; ++i) grad[i] *= wt;                                    \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_FAN_BPT(float, f)
+DEF_FAN_BPT(double, d)
+
+/* ------------------------------------------------------------------ */
+/* Parallel-beam tables: src/bp_par_2d.cc:18-49.
+ * xcos[ia + ix*na] = x*cos(angle), ysin[ia + iy*na] = y*sin(angle). */
+int orc_par_prep(const orc_geom* g, double* xcos, double* ysin)
+{
+    const double centx = ((double)(g->nx - 1)) / 2 + g->offset_x;
+    const double centy = ((double)(g->ny - 1)) / 2 + g->offset_y;
+    double angle = g->orbit_start;
+    unsigned int ia, ix, iy;
+    for (ia = 0; ia < g->na; ++ia) {
+        const double sa = sin(angle), ca = cos(angle);
+        double posx = -centx * g->dx, posy = centy * g->dy;
+        for (ix = 0; ix < g->nx; ++ix) { xcos[ia + ix * g->na] = posx * ca; posx += g->dx; }
+        for (iy = 0; iy < g->ny; ++iy) { ysin[ia + iy * g->na] = posy * sa; posy -= g->dy; }
+        angle += g->orbit;
+    }
+    return 1;
+}
+
+/* Pixel-driven parallel BP, src/bp_par_2d.cc:51-90 (signed orbit scale). */
+#define DEF_PAR_BP(T, SUF)                                                        \
+int orc_par_bp_##SUF(const orc_geom* g, const T* proj, T* img, const double* xcos, \
+                     const double* ysin)                                          \
+{                                                                                 \
+    const double cents = ((double)(g->ns - 1)) / 2 + g->offset_s;                 \
+    unsigned int ix, iy, ia, is1, is2;                                            \
+    for (iy = 0; iy < g->ny; ++iy) {                                              \
+        for (ix = 0; ix < g->nx; ++ix) {                                          \
+            double sum = 0.0;                                                     \
+            for (ia = 0; ia < g->na; ++ia) {                                      \
+                const double s = (xcos[ia + ix * g->na] + ysin[ia + iy * g->na])  \
+                                 / g->ds + cents;                                 \
+                if (s >= 0 && s <= g->ns - 1) {                                   \
+                    double u;                                                     \
+                    is1 = (unsigned int)floor(s);                                 \
+                    is2 = (unsigned int)ceil(s);                                  \
+                    u = s - is1;                                                  \
+                    sum += (1 - u) * proj[is1 + ia * g->ns]                       \
+                           + u * proj[is2 + ia * g->ns];                          \
+                }                                                                 \
+            }                                                                     \
+            img[ix + iy * g->nx] = (T)(sum * g->orbit);                           \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_PAR_BP(float, f)
+DEF_PAR_BP(double, d)
+
+/* Transpose of the parallel BP (scatter form, FP64 accumulation).  The
+ * reference's transpose (src/bp_par_2d.cc:95-199) reads uninitialised
+ * pointers, so this one is the mathematically intended operator, pinned by
+ * the adjoint test against orc_par_bp. */
+#define DEF_PAR_BPT(T, SUF)                                                       \
+int orc_par_bpt_##SUF(const orc_geom* g, const T* img, T* grad, const double* xcos, \
+                      const double* ysin, double* work)                           \
+{                                                                                 \
+    const double cents = ((double)(g->ns - 1)) / 2 + g->offset_s;                 \
+    unsigned int ix, iy, ia, is1, is2;                                            \
+    size_t i, n = (size_t)g->ns * g->na;                                          \
+    for (i = 0; i < n; ++i) work[i] = 0.0;                                        \
+    for (iy = 0; iy < g->ny; ++iy) {                                              \
+        for (ix = 0; ix < g->nx; ++ix) {                                          \
+            const double v = img[ix + iy * g->nx];                                \
+            for (ia = 0; ia < g->na; ++ia) {                                      \
+                const double s = (xcos[ia + ix * g->na] + ysin[ia + iy * g->na])  \
+                                 / g->ds + cents;                                 \
+                if (s >= 0 && s <= g->ns - 1) {                                   \
+                    double u;                                                     \
+                    is1 = (unsigned int)floor(s);                                 \
+                    is2 = (unsigned int)ceil(s);                                  \
+                    u = s - is1;                                                  \
+                    work[is1 + ia * g->ns] += (1 - u) * v;                        \
+                    work[is2 + ia * g->ns] += u * v;                              \
+                }                                                                 \
+            }                                                                     \
+        }                                                                         \
+    }                                                                             \
+    for (i = 0; i < n; ++i) grad[i] = (T)(work[i] * g->orbit);                    \
+    return 1;                                                                     \
+}
+DEF_PAR_BPT(float, f)
+DEF_PAR_BPT(double, d)
+
+/* ------------------------------------------------------------------ */
+/* Ray-driven parallel FP tables, src/fp_par_2d.cc:149-208.
+ * sincostbl[2a]=sin, [2a+1]=cos of the wrapped, accumulated angle;
+ * beginoffset[2(a*ns+s)] = begin + s*offset2, [..+1] = offset;
+ * usex[a] = 0 when angle in [pi/4,3pi/4) U [5pi/4,7pi/4). */
+int orc_fp_prep(const orc_geom* g, double* sincostbl, double* beginoffset, int* usex)
+{
+    const double centx = (double)(g->nx - 1) / 2.0 + g->offset_x;
+    const double centy = (double)(g->ny - 1) / 2.0 + g->offset_y;
+    const double cents = (double)(g->ns - 1) / 2.0 + g->offset_s;
+    double angle = g->orbit_start;
+    unsigned int ia, is;
+    double* bp = beginoffset;
+    for (ia = 0; ia < g->na; ++ia) {
+        double sa, ca, offset, offset2, begin;
+        while (angle < 0) angle += 2 * M_PI;
+        while (angle >= 2 * M_PI) angle -= 2 * M_PI;
+        sa = sin(angle);
+        ca = cos(angle);
+        sincostbl[2 * ia] = sa;
+        sincostbl[2 * ia + 1] = ca;
+        if ((angle >= M_PI_4 && angle < 3 * M_PI_4) ||
+            (angle >= 5 * M_PI_4 && angle < 7 * M_PI_4))
+            usex[ia] = 0;
+        else
+            usex[ia] = 1;
+        if (usex[ia]) {
+            offset = g->dy * sa / (ca * g->dx);
+            offset2 = g->ds / (ca * g->dx);
+            begin = centx - centy * offset - cents * offset2;
+        } else {
+            offset = g->dx * ca / (sa * g->dy);
+            offset2 = -g->ds / (sa * g->dy);
+            begin = centy - centx * offset - cents * offset2;
+        }
+        for (is = 0; is < g->ns; ++is) {
+            bp[0] = begin;
+            bp[1] = offset;
+            begin += offset2;
+            bp += 2;
+        }
+        angle += g->orbit;
+    }
+    return 1;
+}
+
+/* Ray-driven parallel FP, src/fp_par_2d.cc:210-280.  The running sum is
+ * of type T (the reference accumulates in T, :219); positions are FP64
+ * and advanced by repeated addition (:247, :271). */
+#define DEF_FP(T, SUF)                                                            \
+int orc_fp_##SUF(const orc_geom* g, const T* img, T* proj, const double* sc,      \
+                 const double* beginoffset, const int* usex)                      \
+{                                                                                 \
+    unsigned int ia, is, k, i1, i2;                                               \
+    const double* bp = beginoffset;                                               \
+    T* out = proj;                                                                \
+    for (ia = 0; ia < g->na; ++ia) {                                              \
+        if (usex[ia]) {                                                           \
+            const double dw = fabs(g->dy / sc[2 * ia + 1]);                       \
+            for (is = 0; is < g->ns; ++is) {                                      \
+                double x = bp[0];                                                 \
+                const double off = bp[1];                                         \
+                T sum = 0.0;                                                      \
+                for (k = 0; k < g->ny; ++k) {                                     \
+                    if (x >= 0 && x <= g->nx - 1) {                               \
+                        double u;                                                 \
+                        i1 = (unsigned int)floor(x);                              \
+                        i2 = (unsigned int)ceil(x);                               \
+                        u = x - i1;                                               \
+                        sum += (1 - u) * img[i1 + k * g->nx] + u * img[i2 + k * g->nx]; \
+                    }                                                             \
+                    x += off;                                                     \
+                }                                                                 \
+                *out++ = (T)(sum * dw);                                           \
+                bp += 2;                                                          \
+            }                                                                     \
+        } else {                                                                  \
+            const double dw = fabs(g->dx / sc[2 * ia]);                           \
+            for (is = 0; is < g->ns; ++is) {                                      \
+                double y = bp[0];                                                 \
+                const double off = bp[1];                                         \
+                T sum = 0.0;                                                      \
+                for (k = 0; k < g->nx; ++k) {                                     \
+                    if (y >= 0 && y <= g->ny - 1) {                               \
+                        double u;                                                 \
+                        i1 = (unsigned int)floor(y);                              \
+                        i2 = (unsigned int)ceil(y);                               \
+                        u = y - i1;                                               \
+                        sum += (1 - u) * img[k + i1 * g->nx] + u * img[k + i2 * g->nx]; \
+                    }                                                             \
+                    y += off;                                                     \
+                }                                                                 \
+                *out++ = (T)(sum * dw);                                           \
+                bp += 2;                                                          \
+            }                                                                     \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_FP(float, f)
+DEF_FP(double, d)
+
+/* Tables of the FP transpose, src/fp_par_2d.cc:285-330.  No angle wrap
+ * here (the reference's GradPrep does not wrap, :301-307). pos is laid out
+ * [na][nx or ny][2]; the reference indexes it with a stride of 2*nx per view,
+ * so it requires nx == ny (as its TF op enforces). */
+int orc_fp_grad_prep(const orc_geom* g, double* weights, double* pos, int* usex)
+{
+    const double centx = (double)(g->nx - 1) / 2.0 + g->offset_x;
+    const double centy = (double)(g->ny - 1) / 2.0 + g->offset_y;
+    const double cents = (double)(g->ns - 1) / 2.0 + g->offset_s;
+    double angle = g->orbit_start;
+    unsigned int ia, k;
+    double* pp = pos;
+    for (ia = 0; ia < g->na; ++ia) {
+        const double sa = sin(angle), ca = cos(angle);
+        const int bx = !((angle >= M_PI_4 && angle < 3 * M_PI_4) ||
+                         (angle >= 5 * M_PI_4 && angle < 7 * M_PI_4));
+        double o1, o2, begin;
+        usex[ia] = bx;
+        if (bx) {
+            weights[ia] = fabs(g->dy / ca);
+            o1 = g->dx * ca / g->ds;
+            o2 = -g->dy * sa / g->ds;
+            begin = cents - centx * o1 - centy * o2;
+            for (k = 0; k < g->ny; ++k) { pp[0] = begin; pp[1] = o1; begin += o2; pp += 2; }
+        } else {
+            weights[ia] = fabs(g->dx / sa);
+            o1 = -g->dy * sa / g->ds;
+            o2 = g->dx * ca / g->ds;
+            begin = cents - centy * o1 - centx * o2;
+            for (k = 0; k < g->nx; ++k) { pp[0] = begin; pp[1] = o1; begin += o2; pp += 2; }
+        }
+        angle += g->orbit;
+    }
+    return 1;
+}
+
+/* FP transpose in gather form, src/fp_par_2d.cc:332-391.  Differences
+ * from the reference, both where the reference is undefined: the channel
+ * loop is clamped to [0, ns-1] (the reference converts a negative ceil() to
+ * unsigned and reads past the row when right > ns-1, :375-378).  Inside the
+ * detector the result is the reference's. */
+#define DEF_FP_GRAD(T, SUF)                                                       \
+int orc_fp_grad_##SUF(const orc_geom* g, const T* proj, T* img,                   \
+                      const double* weights, const double* pos, const int* usex)  \
+{                                                                                 \
+    unsigned int ix, iy, ia;                                                      \
+    for (iy = 0; iy < g->ny; ++iy) {                                              \
+        for (ix = 0; ix < g->nx; ++ix) {                                          \
+            double sum = 0.0;                                                     \
+            for (ia = 0; ia < g->na; ++ia) {                                      \
+                const T* pr = proj + (size_t)ia * g->ns;                          \
+                double begin, off, mid, left, right, tmp = 0.0, c;                \
+                if (usex[ia]) {                                                   \
+                    const double* pp = pos + 2 * ((size_t)ia * g->nx + iy);       \
+                    begin = pp[0]; off = pp[1];                                   \
+                    mid = begin + off * ix;                                       \
+                    left = ix == 0 ? mid : mid - off;                             \
+                    right = ix == g->nx - 1 ? mid : mid + off;                    \
+                } else {                                                          \
+                    const double* pp = pos + 2 * ((size_t)ia * g->nx + ix);       \
+                    begin = pp[0]; off = pp[1];                                   \
+                    mid = begin + off * iy;                                       \
+                    left = iy == 0 ? mid : mid - off;                             \
+                    right = iy == g->ny - 1 ? mid : mid + off;                    \
+                }                                                                 \
+                if (left > right) { double t = left; left = right; right = t; }   \
+                c = ceil(left);                                                   \
+                if (c < 0) c = 0;                                                 \
+                for (; c <= right && c <= (double)(g->ns - 1); c += 1.0) {        \
+                    const unsigned int is = (unsigned int)c;                      \
+                    tmp += (1 - fabs((mid - (double)is) / off)) * pr[is];         \
+                }                                                                 \
+                sum += tmp * weights[ia];                                         \
+            }                                                                     \
+            img[ix + iy * g->nx] = (T)sum;                                        \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_FP_GRAD(float, f)
+DEF_FP_GRAD(double, d)
+
+/* ------------------------------------------------------------------ */
+/* Ramp filter taps filter[ns + n], n in [-ns, ns], src/filter.cc:18-65.
+ * Returns 0 for an unknown type (:62). */
+int orc_ramp_prep(const orc_geom* g, double* filter)
+{
+    double* pf = filter + g->ns;
+    int n, ns = (int)g->ns;
+    if (g->type == 0 || g->type == 2) {
+        const double ds2_inv = 1.0 / (g->ds * g->ds);
+        const double pi2_inv = 1.0 / (M_PI * M_PI);
+        for (n = -ns; n <= ns; ++n) {
+            if (n == 0) pf[n] = 0.25 * ds2_inv;
+            else if (n % 2 == 0) pf[n] = 0;
+            else pf[n] = -pi2_inv * ds2_inv / (double)(n * n);
+        }
+    } else if (g->type == 1) {
+        const double k = 1.0 / (M_PI * M_PI * g->dsd * g->dsd);
+        for (n = -ns; n <= ns; ++n) {
+            if (n == 0) pf[n] = 0.25 / (g->ds * g->ds);
+            else if (n % 2 == 0) pf[n] = 0;
+            else {
+                const double sa = sin((double)n * g->ds / g->dsd);
+                pf[n] = -k / (sa * sa);
+            }
+        }
+    } else {
+        return 0;
+    }
+    return 1;
+}
+
+/* Linear convolution per row, src/filter.cc:67-91 (FP64 sum, all taps). */
+#define DEF_RAMP(T, SUF)                                                          \
+int orc_ramp_##SUF(const orc_geom* g, const T* in, const double* filter, T* out)  \
+{                                                                                 \
+    const double* pf = filter + g->ns;                                            \
+    const int ns = (int)g->ns;                                                    \
+    unsigned int row;                                                             \
+    int is, n;                                                                    \
+    for (row = 0; row < g->nrow; ++row) {                                         \
+        const T* ip = in + (size_t)row * g->ns;                                   \
+        T* op = out + (size_t)row * g->ns;                                        \
+        for (is = 0; is < ns; ++is) {                                             \
+            double sum = 0;                                                       \
+            for (n = -ns; n <= ns; ++n) {                                         \
+                const int k = is - n;                                             \
+                if (k >= 0 && k < ns) sum += ip[k] * pf[n];                       \
+            }                                                                     \
+            op[is] = (T)(sum * g->ds);                                            \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_RAMP(float, f)
+DEF_RAMP(double, d)
+
+/* Correlation (the transpose), src/filter.cc:96-124; zero taps skipped. */
+#define DEF_RAMP_GRAD(T, SUF)                                                     \
+int orc_ramp_grad_##SUF(const orc_geom* g, const T* in, const double* filter, T* out) \
+{                                                                                 \
+    const double* pf = filter + g->ns;                                            \
+    const int ns = (int)g->ns;                                                    \
+    unsigned int row;                                                             \
+    int is, n;                                                                    \
+    for (row = 0; row < g->nrow; ++row) {                                         \
+        const T* ip = in + (size_t)row * g->ns;                                   \
+        T* op = out + (size_t)row * g->ns;                                        \
+        for (is = 0; is < ns; ++is) {                                             \
+            double sum = 0;                                                       \
+            for (n = -ns; n <= ns; ++n) {                                         \
+                const int k = is + n;                                             \
+                if (k >= 0 && k < ns && (pf[n] > 0 || pf[n] < 0))                 \
+                    sum += ip[k] * pf[n];                                         \
+            }                                                                     \
+            op[is] = (T)(sum * g->ds);                                            \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_RAMP_GRAD(float, f)
+DEF_RAMP_GRAD(double, d)
+
+/* ------------------------------------------------------------------ */
+/* Fan / flat cosine weighting, src/fan_weighting.cc:19-59.  The transpose
+ * is the same diagonal (:39-56), so one routine serves both (grad=1 only
+ * changes the operand order of the product, as the reference does). */
+#define DEF_FANW(T, SUF)                                                          \
+int orc_fanw_##SUF(const orc_geom* g, const T* in, T* out, int grad)              \
+{                                                                                 \
+    const double cents = (double)(g->ns - 1) / 2 + g->offset_s;                   \
+    int is, ia;                                                                   \
+    for (is = 0; is < (int)g->ns; ++is) {                                         \
+        const double s = g->ds * ((double)is - cents);                            \
+        double w;                                                                 \
+        if (g->type == 2) w = g->dso * fabs(cos(atan2(s, g->dsd))) / g->dsd;      \
+        else if (g->type == 1) w = g->dso * fabs(cos(s / g->dsd)) / g->dsd;       \
+        else return 0;                                                            \
+        for (ia = 0; ia < (int)g->nrow; ++ia) {                                   \
+            const size_t i = is + (size_t)ia * g->ns;                             \
+            out[i] = grad ? (T)(in[i] * w) : (T)(w * in[i]);                      \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_FANW(float, f)
+DEF_FANW(double, d)

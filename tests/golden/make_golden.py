@@ -1,0 +1,165 @@
+"""Write golden vectors for the 2-D CT hot path from the compiled reference.
+
+Run in a container that has the reference checkout:
+
+    make -C oracle ref          # builds oracle/_ref/libctref.so from src/*.cc
+    python tests/golden/make_golden.py
+
+Every fixture is an ``.npz`` of inputs and the reference's outputs (its own
+``calculate_on_cpu`` through ``oracle/ref_shim.cpp``).  Large inputs are not
+stored: they are regenerated from the recorded seed with
+``numpy.random.default_rng(seed).standard_normal(shape)``, the same call the
+tests make.  Geometries are chosen so that every edge rule of the reference
+is exercised (pixels projecting outside the detector, offsets on every axis,
+the pi/4 branch boundary of the ray-driven FP under accumulated angles) while
+staying inside the region where the reference is well defined.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
+
+from oracle import oracle as O  # noqa: E402
+
+
+def randn(seed, shape, dtype=np.float32):
+    return np.random.default_rng(seed).standard_normal(shape).astype(dtype)
+
+
+def gdict(g):
+    return {k: getattr(g, k) for k, _ in O.Geom._fields_}
+
+
+def save(name, g, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, geom=json.dumps(gdict(g)), **arrays)
+    print(f"wrote {path} ({os.path.getsize(path)} B)")
+
+
+# Geometries ---------------------------------------------------------------
+# Small fan geometry: part of the image projects off the detector.
+FAN_SMALL = dict(ns=41, na=48, ds=1.7, orbit=2 * math.pi / 48, offset_s=0.4,
+                 orbit_start=0.3, nx=24, ny=20, dx=1.3, dy=1.5, offset_x=0.3,
+                 offset_y=-0.2, dso=60.0, dsd=110.0)
+# The 512x512 / 720-view fan geometry (SID 595, SDD 1085.6, 736 channels at
+# 1.30 mm arc pitch) sampled on a coarse 64x64 grid covering the same FOV.
+FAN_C2_COARSE = dict(ns=736, na=720, ds=1.30, orbit=2 * math.pi / 720, offset_s=0.0,
+                     orbit_start=0.0, nx=64, ny=64, dx=0.7 * 8, dy=0.7 * 8,
+                     offset_x=0.0, offset_y=0.0, dso=595.0, dsd=1085.6)
+PAR_SMALL = dict(ns=41, na=36, ds=0.8, orbit=math.pi / 36, offset_s=-0.35,
+                 orbit_start=0.1, nx=22, ny=26, dx=0.7, dy=0.9, offset_x=-0.4,
+                 offset_y=0.25)
+# Ray-driven FP: nx == ny (the reference's transpose requires it); the
+# detector covers the image so the reference's transpose stays defined.
+FP_SMALL = dict(ns=41, na=40, ds=1.0, orbit=math.pi / 40, offset_s=0.3,
+                orbit_start=0.0, nx=24, ny=24, dx=1.0, dy=1.0, offset_x=0.2,
+                offset_y=-0.1)
+# 360 views over 180 deg (view 270 sits on 3pi/4) and 720 views over 180 deg
+# (view 180 sits on pi/4): the branch is decided on the accumulated angle.
+FP_360 = dict(ns=25, na=360, ds=1.0, orbit=math.pi / 360, offset_s=0.0,
+              orbit_start=0.0, nx=16, ny=16, dx=1.0, dy=1.0)
+FP_720 = dict(ns=25, na=720, ds=1.0, orbit=math.pi / 720, offset_s=0.0,
+              orbit_start=0.0, nx=16, ny=16, dx=1.0, dy=1.0)
+
+
+def fan_fixtures():
+    g = O.geom(**FAN_SMALL)
+    t = O.fan_prep(g, "ref")
+    proj = randn(11, (g.na, g.ns))
+    img = randn(12, (g.ny, g.nx))
+    save("fan_small", g, proj=proj, img=img, xpos=t[0], ypos=t[1], sincostbl=t[2],
+         bp_f32=O.fan_bp(g, proj, t, kind="ref"),
+         bp_f64=O.fan_bp(g, proj.astype(np.float64), t, kind="ref"),
+         bpt_f32=O.fan_bpt(g, img, t, kind="ref"),
+         bpt_f64=O.fan_bpt(g, img.astype(np.float64), t, kind="ref"))
+
+    g = O.geom(**FAN_C2_COARSE)
+    t = O.fan_prep(g, "ref")
+    proj = randn(21, (g.na, g.ns))
+    img = randn(22, (g.ny, g.nx))
+    save("fan_c2_coarse", g, proj_seed=np.int64(21), img=img, sincostbl=t[2],
+         bp_f32=O.fan_bp(g, proj, t, kind="ref"),
+         bp_f64=O.fan_bp(g, proj.astype(np.float64), t, kind="ref"),
+         bpt_f32_every60=O.fan_bpt(g, img, t, kind="ref")[::60].copy())
+
+    # Irregular, sorted angles (sparse-view style subset plus jitter).
+    g = O.geom(**{**FAN_SMALL, "na": 37})
+    rng = np.random.default_rng(31)
+    angles = np.sort(rng.uniform(0, 2 * math.pi, g.na))
+    t = O.fan_angle_prep(g, angles, "ref")
+    proj = randn(32, (g.na, g.ns))
+    save("fan_angle_small", g, angles=angles, proj=proj, xpos=t[0], ypos=t[1],
+         sincostbl=t[2],
+         bp_f32=O.fan_bp(g, proj, t, scaled=False, kind="ref"),
+         bp_f64=O.fan_bp(g, proj.astype(np.float64), t, scaled=False, kind="ref"))
+
+
+def par_fixtures():
+    g = O.geom(**PAR_SMALL)
+    t = O.par_prep(g, "ref")
+    proj = randn(41, (g.na, g.ns))
+    save("par_small", g, proj=proj, xcos=t[0], ysin=t[1],
+         bp_f32=O.par_bp(g, proj, t, kind="ref"),
+         bp_f64=O.par_bp(g, proj.astype(np.float64), t, kind="ref"))
+
+
+def fp_fixtures():
+    g = O.geom(**FP_SMALL)
+    t = O.fp_prep(g, "ref")
+    tg = O.fp_grad_prep(g, "ref")
+    img = randn(51, (g.ny, g.nx))
+    proj = randn(52, (g.na, g.ns))
+    save("fp_small", g, img=img, proj=proj, sincostbl=t[0], beginoffset=t[1], usex=t[2],
+         weights=tg[0], pos=tg[1], usex_grad=tg[2],
+         fp_f32=O.fp(g, img, t, kind="ref"),
+         fp_f64=O.fp(g, img.astype(np.float64), t, kind="ref"),
+         grad_f32=O.fp_grad(g, proj, tg, kind="ref"),
+         grad_f64=O.fp_grad(g, proj.astype(np.float64), tg, kind="ref"))
+    for name, geo, seed in (("fp_360", FP_360, 61), ("fp_720", FP_720, 62)):
+        g = O.geom(**geo)
+        t = O.fp_prep(g, "ref")
+        img = randn(seed, (g.ny, g.nx))
+        save(name, g, img=img, usex=t[2], fp_f32=O.fp(g, img, t, kind="ref"),
+             fp_f64=O.fp(g, img.astype(np.float64), t, kind="ref"))
+
+
+def filter_fixtures():
+    out = {}
+    for typ in (0, 1, 2):
+        g = O.geom(ns=33, nrow=5, ds=0.9, dsd=40.0, type=typ)
+        h = O.ramp_prep(g, "ref")
+        x = randn(70 + typ, (g.nrow, g.ns))
+        out[f"taps_{typ}"] = h
+        out[f"in_{typ}"] = x
+        out[f"out_f32_{typ}"] = O.ramp(g, x, h, kind="ref")
+        out[f"out_f64_{typ}"] = O.ramp(g, x.astype(np.float64), h, kind="ref")
+        out[f"grad_f32_{typ}"] = O.ramp(g, x, h, grad=True, kind="ref")
+        out[f"grad_f64_{typ}"] = O.ramp(g, x.astype(np.float64), h, grad=True, kind="ref")
+    save("ramp_small", O.geom(ns=33, nrow=5, ds=0.9, dsd=40.0), **out)
+
+    out = {}
+    for typ in (1, 2):
+        g = O.geom(ns=37, nrow=4, ds=1.3, offset_s=0.6, dso=595.0, dsd=1085.6, type=typ)
+        x = randn(80 + typ, (g.nrow, g.ns))
+        out[f"in_{typ}"] = x
+        out[f"out_f32_{typ}"] = O.fanw(g, x, kind="ref")
+        out[f"out_f64_{typ}"] = O.fanw(g, x.astype(np.float64), kind="ref")
+        out[f"grad_f32_{typ}"] = O.fanw(g, x, grad=True, kind="ref")
+    save("fanw_small", O.geom(ns=37, nrow=4, ds=1.3, offset_s=0.6, dso=595.0, dsd=1085.6),
+         **out)
+
+
+if __name__ == "__main__":
+    if not O.have_ref():
+        sys.exit("oracle/_ref/libctref.so missing: run `make -C oracle ref` first")
+    fan_fixtures()
+    par_fixtures()
+    fp_fixtures()
+    filter_fixtures()
