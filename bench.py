@@ -1,0 +1,266 @@
+#!/usr/bin/env python3
+"""Benchmark of the fan-beam FBP hot path on one MI355X.
+
+Workload (one "step"): a batch of 16 seeded-ellipse 512x512 slices, equiangular
+fan, 720 views over 360 deg, 736 channels at 1.30 mm arc pitch, SID 595 mm,
+SDD 1085.6 mm, pixel 0.7 mm; analytic ellipse line integrals plus Poisson noise
+at 1e5 photons; the step is fan weighting fused into the Ram-Lak ramp filter,
+then bp_fan_2d, all on inputs already resident in HBM.
+
+Prints one JSON line: whole-batch throughput in Gpixel*view/s (one
+pixel*view = one interpolate-accumulate of one pixel from one view), the
+roofline of the dominant kernel (the BP) timed with HIP events on the stream
+it runs on, the reference's own CPU path (oracle/_ref, else the C
+restatement) timed on this host's cores on a bounded sample, the relative L2
+error against the reference's float and FP64 CPU results, and the plain
+per-slice GPU path (the reference's execution structure) for comparison.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X published peaks
+PEAK_HBM_GBS = 8000.0
+CU, LANES, CLK = 256, 64, 2.4e9
+PEAK_VALU_SLOTS = CU * LANES * CLK  # FP32 VALU lane-issues per second (3.93e13)
+
+# Algorithmic VALU issue slots of the fan BP: geometry per
+# pixel*view, shared by every slice of a batch group, and per slice.
+FAN_GEOM_SLOTS = 23
+FAN_SLICE_SLOTS = 3
+SURVEY_FAN_SLOTS = 28  # per-slice model of the survey (no batching)
+
+CFG = dict(nbatch=16, n=512, dx=0.7, na=720, ns=736, ds=1.30, dso=595.0, dsd=1085.6,
+           i0=1e5, ellipses=10, seed=20261202)
+
+
+def make_inputs(cfg):
+    from ct_recon_extensions_amd import phantom as P
+    orbit = 2 * math.pi / cfg["na"]
+    radius = cfg["n"] * cfg["dx"] / 2
+    sinos = np.empty((cfg["nbatch"], cfg["na"], cfg["ns"]), np.float32)
+    for b in range(cfg["nbatch"]):
+        ell = P.random_ellipses(cfg["ellipses"], cfg["seed"] + b, radius=radius)
+        p = P.fan_sinogram(ell, cfg["na"], cfg["ns"], cfg["ds"], orbit, cfg["dso"], cfg["dsd"])
+        sinos[b] = P.poisson_noise(p, cfg["i0"], cfg["seed"] + 1000 + b)
+    return sinos
+
+
+def kw_fan(cfg):
+    return dict(img_shape=[cfg["n"], cfg["n"]], img_space=[cfg["dx"], cfg["dx"]],
+                proj_shape=[cfg["na"], cfg["ns"]], channel_space=cfg["ds"],
+                orbit=2 * math.pi / cfg["na"], dso=cfg["dso"], dsd=cfg["dsd"])
+
+
+def kw_filter(cfg):
+    return dict(proj_shape=[cfg["na"], cfg["ns"]], channel_space=cfg["ds"], dso=cfg["dso"],
+                dsd=cfg["dsd"], tp="fan")
+
+
+def time_events(fn, reps, torch):
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps  # ms
+
+
+def cpu_baseline(cfg, sinos, threads, budget_s=30.0):
+    """Reference CPU FBP (fan weighting -> ramp -> bp_fan_2d), one slice per
+    thread, on `threads` host threads (ctypes releases the GIL).  Slice 0 is
+    run both in float (the reference's float path) and in double (FP64
+    oracle) for the error figures."""
+    from oracle import oracle as O
+    kind = "ref" if O.have_ref() else "oracle"
+    g = O.geom(ns=cfg["ns"], na=cfg["na"], ds=cfg["ds"], orbit=2 * math.pi / cfg["na"],
+               nx=cfg["n"], ny=cfg["n"], dx=cfg["dx"], dy=cfg["dx"], dso=cfg["dso"],
+               dsd=cfg["dsd"], type=1, nrow=cfg["na"])
+    tables = O.fan_prep(g, kind)
+    taps = O.ramp_prep(g, kind)
+
+    def fbp(x):
+        return O.fan_bp(g, O.ramp(g, O.fanw(g, x, kind=kind), taps, kind=kind), tables, kind=kind)
+
+    jobs = [sinos[0], sinos[0].astype(np.float64)]
+    jobs += [sinos[b % len(sinos)] for b in range(1, threads - 1)]
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        outs = list(ex.map(fbp, jobs))
+    wall = time.perf_counter() - t0
+    pv = len(jobs) * cfg["n"] ** 2 * cfg["na"]
+    return dict(kind="reference" if kind == "ref" else "port", wall_s=wall, slices=len(jobs),
+                pixel_views=pv, ref_f32=outs[0], ref_f64=outs[1])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads for the CPU baseline (default: min(16, cores))")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-slice GPU path")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this path shards by slice with no exchange, so "
+              "multi-GPU throughput is N x the one-GPU figure; run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ct_recon_extensions_amd as F
+    from ct_recon_extensions_amd import _lib
+    from ct_recon_extensions_amd.geometry import FanBackprojection2DParam, device_tables
+
+    lib = _lib.load()
+    cfg = dict(CFG)
+    dev = torch.device("cuda:0")
+    t_gen = time.perf_counter()
+    sinos = make_inputs(cfg)
+    t_gen = time.perf_counter() - t_gen
+    print(f"[bench] inputs generated in {t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    proj = torch.from_numpy(sinos).to(dev)
+    kwf, kwb = kw_filter(cfg), kw_fan(cfg)
+    filt = F.fan_filter(proj, **kwf)
+    img = F.fan_backprojection_2d(filt, **kwb)
+
+    def step():
+        f = F.fan_filter(proj, **kwf)
+        return F.fan_backprojection_2d(f, **kwb)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ms_step = time_events(step, args.steps, torch)
+    print(f"[bench] step {ms_step:.3f} ms", file=sys.stderr, flush=True)
+
+    # per-kernel durations on the stream the kernels are launched on
+    ms_filter = time_events(lambda: F.fan_filter(proj, **kwf), args.steps, torch)
+    ms_bp = time_events(lambda: F.fan_backprojection_2d(filt, **kwb), args.steps, torch)
+
+    B, n, na, ns = cfg["nbatch"], cfg["n"], cfg["na"], cfg["ns"]
+    pv_slice = n * n * na
+    pv_batch = B * pv_slice
+    value = pv_batch / (ms_step * 1e-3) / 1e9
+    bp_rate = pv_batch / (ms_bp * 1e-3)
+    nb_group = 16
+    groups = math.ceil(B / nb_group)
+    slots = pv_slice * FAN_GEOM_SLOTS * groups + pv_batch * FAN_SLICE_SLOTS
+    valu_achieved = slots / (ms_bp * 1e-3)
+    hbm_bytes = B * 4 * (na * ns + n * n) + 16 * na + 8 * 2 * n
+    hbm_achieved = hbm_bytes / (ms_bp * 1e-3) / 1e9
+
+    # plain per-slice GPU path: one launch per slice, FP64, stream sync per slice
+    plain = None
+    if not args.no_plain:
+        p = FanBackprojection2DParam(ns, na, cfg["ds"], 2 * math.pi / na, 0.0, 0.0, n, n,
+                                     cfg["dx"], cfg["dx"], 0.0, 0.0, cfg["dso"], cfg["dsd"])
+        xpos, ypos, sc = device_tables("fan", p, dev)
+        out_plain = torch.empty_like(img)
+        g = p.ctr()
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+        def plain_bp():
+            _lib.check(lib.ctr_plain_fan_bp(C.byref(g), 0, C.c_void_p(filt.data_ptr()),
+                                            C.c_void_p(out_plain.data_ptr()), B,
+                                            C.c_void_p(xpos.data_ptr()), C.c_void_p(ypos.data_ptr()),
+                                            C.c_void_p(sc.data_ptr()), p.scale, st), "plain")
+
+        plain_bp()
+        torch.cuda.synchronize()
+        ms_plain = time_events(plain_bp, 2, torch)
+        plain_err = float((out_plain - img).double().norm() / out_plain.double().norm())
+        plain = dict(ms_per_batch_bp=ms_plain, slices_per_s_bp=B / (ms_plain * 1e-3),
+                     speedup_bp=ms_plain / ms_bp, rel_l2_fast_vs_plain=plain_err)
+        print(f"[bench] plain BP {ms_plain:.2f} ms/batch", file=sys.stderr, flush=True)
+
+    result = {
+        "metric": "fan-beam FBP backprojection throughput",
+        "value": round(value, 3),
+        "unit": "Gpixel*view/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded analytic ellipse fan sinograms + Poisson 1e5)",
+        "config": {"workload": "fan FBP: 16 x 512^2 slices, 720 views/360deg, 736 ch, "
+                               "SID 595, SDD 1085.6, dx 0.7, ds 1.30; fan weighting + "
+                               "Ram-Lak + bp_fan_2d",
+                   "global_batch": B, "parallelism": "single GPU (slices batched per launch)"},
+        "slices_per_s": round(B / (ms_step * 1e-3), 1),
+        "kernels_ms": {"fan_filter": round(ms_filter, 4), "fan_bp": round(ms_bp, 4)},
+        "roofline": {
+            "kernel": "fan_bp_kernel",
+            "bound": "valu",
+            "achieved": round(valu_achieved / 1e12, 3),
+            "peak": round(PEAK_VALU_SLOTS / 1e12, 3),
+            "unit": "Tslot/s (FP32 VALU lane-issues)",
+            "frac": round(valu_achieved / PEAK_VALU_SLOTS, 4),
+            "slots_per_launch": slots,
+            "traffic": None,
+            "hbm": {"achieved": round(hbm_achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(hbm_achieved / PEAK_HBM_GBS, 5),
+                    "algorithmic_bytes": hbm_bytes},
+            "bp_pixel_views_per_s": bp_rate,
+            "survey_28slot_model_frac": round(bp_rate * SURVEY_FAN_SLOTS / PEAK_VALU_SLOTS, 4),
+        },
+        "plain_per_slice_gpu": plain,
+    }
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = max(2, threads)
+        print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)
+        cb = cpu_baseline(cfg, sinos, threads)
+        got = img[0].double().cpu().numpy()
+        e32 = float(np.linalg.norm(got - cb["ref_f32"]) / np.linalg.norm(cb["ref_f32"]))
+        e64 = float(np.linalg.norm(got - cb["ref_f64"]) / np.linalg.norm(cb["ref_f64"]))
+        cpu_rate = cb["pixel_views"] / cb["wall_s"] / 1e9
+        result["cpu_baseline"] = {
+            "value": round(cpu_rate, 5), "unit": "Gpixel*view/s", "cores": threads,
+            "kind": cb["kind"],
+            "sample": f"{cb['slices']} full slices of the same workload (FBP incl. weighting "
+                      f"and ramp), one per thread, {cb['wall_s']:.1f}s wall",
+            "host_cpu": _cpu_model(), "nproc": os.cpu_count(),
+        }
+        result["vs_cpu"] = round(value / cpu_rate, 1)
+        result["rel_l2_vs_reference_f32"] = e32
+        result["rel_l2_vs_fp64"] = e64
+
+    print(json.dumps(result))
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

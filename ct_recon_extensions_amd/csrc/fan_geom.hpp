@@ -1,0 +1,111 @@
+// Per-(pixel tile, view) fan-beam geometry shared by the BP kernel and its
+// transpose, so both evaluate bit-identical channel positions and weights.
+//
+// For a pixel at (x, y) and source angle beta (source at
+// (-dso sin b, dso cos b)) the reference computes, in FP64
+// (src/bp_fan_2d.cc:66-70):
+//     d = dso + x sin b - y cos b,  r = x cos b + y sin b,
+//     s = atan2(r, d) * dsd/ds + cents,  w = dsd^2 / (d^2 + r^2).
+// Here the tile anchor (xc, yc) is evaluated once in FP64 and every pixel of
+// the tile is expressed relative to it.  With (dd, dr) the pixel's offsets
+// of (d, r) from the anchor's (dc, rc) and Lc^2 = dc^2 + rc^2,
+//     tan(gamma - gamma_c) = u = (dr dc - dd rc) / (Lc^2 + dd dc + dr rc),
+// which is a ratio of two functions linear in the pixel offsets (i, j), and
+//     d^2 + r^2 = Lc^2 (1 + e)^2 (1 + u^2),  e = (dd dc + dr rc)/Lc^2.
+// So s = s_c + (dsd/ds) atan(u) and w = (dsd^2/Lc^2) / ((1+e)^2 (1+u^2)),
+// evaluated per pixel in FP32 on small numbers: the channel position keeps
+// ~1e-6 channel accuracy where a direct FP32 evaluation of s ~ 700 would
+// lose ~5e-5.  The FP64 anchor carries the large part (s_c) exactly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ctr {
+
+struct alignas(16) FanAnchor {
+    float ai, aj;  // numerator slopes per pixel step in x / y, pre-scaled
+    float bi, bj;  // denominator slopes per pixel step
+    float b0;      // denominator constant
+    float fc;      // anchor channel position minus is0
+    int is0;       // first channel of the staged detector span
+    int inside;    // 1: every pixel of the tile lands strictly inside [0, ns-1]
+};
+
+// Geometry of one fan problem as the kernels see it.
+struct FanProblem {
+    double dso, dsd, ds, cents, dx, dy;
+    int ns, na, nx, ny;
+    // polynomial coefficients in v = k*u (k = dsd/ds):
+    //   atan:  ds(v) = v * (1 + v2*(p1 + v2*p2)),   p1 = -1/(3k^2), p2 = 1/(5k^4)
+    //   1/(1+u^2) = 1 + v2*(g1 + v2*g2),             g1 = -1/k^2,   g2 = 1/k^4
+    float p1, p2, g1, g2;
+    float k;  // dsd/ds in FP32 (exact-mode only)
+};
+
+// Anchor of the tile whose local pixel offsets span [ilo, ihi] x [jlo, jhi]
+// around (xc, yc) = (xpos[ic], ypos[jc]).  FP64, then rounded to FP32.
+__device__ inline FanAnchor fan_anchor(const FanProblem& P, double xc, double yc, double sb,
+                                       double cb, int ilo, int ihi, int jlo, int jhi)
+{
+    const double d = P.dso + xc * sb - yc * cb;
+    const double r = xc * cb + yc * sb;
+    const double L2 = d * d + r * r;
+    const double k = P.dsd / P.ds;
+    const double sc = atan2(r, d) * k + P.cents;
+    // dd = i*dx*sb + j*dy*cb,  dr = i*dx*cb - j*dy*sb  (y decreases with iy)
+    const double nI = P.dx * (cb * d - sb * r) / L2;
+    const double nJ = -P.dy * (sb * d + cb * r) / L2;
+    const double eI = P.dx * (sb * d + cb * r) / L2;
+    const double eJ = P.dy * (cb * d - sb * r) / L2;
+    // The channel offset is monotone along any line, so the tile's extreme
+    // channels are reached at its corners.
+    double lo = 0.0, hi = 0.0;
+    const int ci[2] = {ilo, ihi}, cj[2] = {jlo, jhi};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const double u = (nI * ci[a] + nJ * cj[b]) / (1.0 + eI * ci[a] + eJ * cj[b]);
+            const double dsv = k * atan(u);
+            lo = fmin(lo, dsv);
+            hi = fmax(hi, dsv);
+        }
+    const double smin = sc + lo, smax = sc + hi;
+    const int is0 = static_cast<int>(floor(smin)) - 1;
+    const double q = P.dsd / sqrt(L2);  // sqrt(dsd^2 / Lc^2)
+    FanAnchor A;
+    A.ai = static_cast<float>(k * nI / q);
+    A.aj = static_cast<float>(k * nJ / q);
+    A.bi = static_cast<float>(eI / q);
+    A.bj = static_cast<float>(eJ / q);
+    A.b0 = static_cast<float>(1.0 / q);
+    A.fc = static_cast<float>(sc - is0);
+    A.is0 = is0;
+    A.inside = (smin >= 0.25 && smax <= P.ns - 1.25) ? 1 : 0;
+    return A;
+}
+
+// Channel position (relative to is0) and weight of local pixel (fi, fj).
+// Default mode: short series (the host checks |u| stays small enough);
+// EXACT: atan and 1/(1+u^2) evaluated directly.
+template <bool EXACT>
+__device__ __forceinline__ void fan_pixel(const FanProblem& P, const FanAnchor& A, float fi,
+                                          float fj, float& f, float& w)
+{
+    const float num = fmaf(A.ai, fi, A.aj * fj);
+    const float den = fmaf(A.bi, fi, fmaf(A.bj, fj, A.b0));
+    const float rd = __builtin_amdgcn_rcpf(den);
+    const float v = num * rd;  // k * u, channels
+    if constexpr (EXACT) {
+        const float u = v / P.k;
+        f = A.fc + P.k * atanf(u);
+        w = rd * rd / fmaf(u, u, 1.0f);
+    } else {
+        const float v2 = v * v;
+        const float t = v2 * fmaf(v2, P.p2, P.p1);
+        f = A.fc + fmaf(v, t, v);
+        w = rd * rd * fmaf(v2, fmaf(v2, P.g2, P.g1), 1.0f);
+    }
+}
+
+}  // namespace ctr

@@ -1,0 +1,225 @@
+// FP64 geometry tables, built on the host in the reference's layouts and in
+// its accumulation order (angles advanced by repeated addition, pixel
+// positions likewise), so the device kernels see exactly the values the
+// reference's CPU path uses.  This is the only FP64 precomputation of the
+// path; the kernels round what they need to FP32.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "ctr_common.hpp"
+
+namespace ctr {
+
+static thread_local std::string t_last_error;
+
+void set_error(const std::string& msg) { t_last_error = msg; }
+const char* get_error() { return t_last_error.c_str(); }
+
+}  // namespace ctr
+
+using ctr::cent_s;
+using ctr::cent_x;
+using ctr::cent_y;
+
+extern "C" {
+
+const char* ctr_last_error(void) { return ctr::get_error(); }
+
+int ctr_version(void) { return 10000 * 0 + 100 * 3 + 0; }
+
+// Pixel centres x_ix = -centx*dx + ix*dx and y_iy = centy*dy - iy*dy, by
+// accumulation (src/bp_fan_2d.cc:27-42).
+static void pixel_positions(const ctr_geom& g, double* xpos, double* ypos)
+{
+    double px = -cent_x(g) * g.dx, py = cent_y(g) * g.dy;
+    for (unsigned int i = 0; i < g.nx; ++i) {
+        xpos[i] = px;
+        px += g.dx;
+    }
+    for (unsigned int i = 0; i < g.ny; ++i) {
+        ypos[i] = py;
+        py -= g.dy;
+    }
+}
+
+int ctr_fan_tables(const ctr_geom* g, double* xpos, double* ypos, double* sincostbl)
+{
+    CTR_REQUIRE(g && xpos && ypos && sincostbl, CTR_ERR_ARG, "ctr_fan_tables: null argument");
+    double angle = g->orbit_start;
+    for (unsigned int a = 0; a < g->na; ++a) {
+        sincostbl[2 * a] = std::sin(angle);
+        sincostbl[2 * a + 1] = std::cos(angle);
+        angle += g->orbit;
+    }
+    pixel_positions(*g, xpos, ypos);
+    return CTR_OK;
+}
+
+int ctr_fan_angle_tables(const ctr_geom* g, const double* angles, double* xpos, double* ypos,
+                         double* sincostbl)
+{
+    CTR_REQUIRE(g && angles && xpos && ypos && sincostbl, CTR_ERR_ARG,
+                "ctr_fan_angle_tables: null argument");
+    for (unsigned int a = 0; a < g->na; ++a) {
+        sincostbl[2 * a] = std::sin(angles[a]);
+        sincostbl[2 * a + 1] = std::cos(angles[a]);
+    }
+    pixel_positions(*g, xpos, ypos);
+    return CTR_OK;
+}
+
+// xcos[ix*na + ia] = x*cos(angle), ysin[iy*na + ia] = y*sin(angle)
+// (src/bp_par_2d.cc:18-49).
+int ctr_par_tables(const ctr_geom* g, double* xcos, double* ysin)
+{
+    CTR_REQUIRE(g && xcos && ysin, CTR_ERR_ARG, "ctr_par_tables: null argument");
+    double angle = g->orbit_start;
+    for (unsigned int a = 0; a < g->na; ++a) {
+        const double sa = std::sin(angle), ca = std::cos(angle);
+        double px = -cent_x(*g) * g->dx;
+        for (unsigned int i = 0; i < g->nx; ++i) {
+            xcos[a + static_cast<size_t>(i) * g->na] = px * ca;
+            px += g->dx;
+        }
+        double py = cent_y(*g) * g->dy;
+        for (unsigned int i = 0; i < g->ny; ++i) {
+            ysin[a + static_cast<size_t>(i) * g->na] = py * sa;
+            py -= g->dy;
+        }
+        angle += g->orbit;
+    }
+    return CTR_OK;
+}
+
+// Ray-driven FP tables (src/fp_par_2d.cc:149-208): the angle is wrapped into
+// [0, 2pi) before use, the x-branch is used outside [pi/4,3pi/4) and
+// [5pi/4,7pi/4), and each channel's start position advances by repeated
+// addition.
+int ctr_fp_tables(const ctr_geom* g, double* sincostbl, double* beginoffset, int* usex)
+{
+    CTR_REQUIRE(g && sincostbl && beginoffset && usex, CTR_ERR_ARG, "ctr_fp_tables: null argument");
+    const double cx = cent_x(*g), cy = cent_y(*g), cs = cent_s(*g);
+    const double q = M_PI / 4;
+    double angle = g->orbit_start;
+    double* bo = beginoffset;
+    for (unsigned int a = 0; a < g->na; ++a) {
+        while (angle < 0) angle += 2 * M_PI;
+        while (angle >= 2 * M_PI) angle -= 2 * M_PI;
+        const double sa = std::sin(angle), ca = std::cos(angle);
+        sincostbl[2 * a] = sa;
+        sincostbl[2 * a + 1] = ca;
+        const bool ybranch = (angle >= q && angle < 3 * q) || (angle >= 5 * q && angle < 7 * q);
+        usex[a] = ybranch ? 0 : 1;
+        double step, chstep, begin;
+        if (!ybranch) {
+            step = g->dy * sa / (ca * g->dx);
+            chstep = g->ds / (ca * g->dx);
+            begin = cx - cy * step - cs * chstep;
+        } else {
+            step = g->dx * ca / (sa * g->dy);
+            chstep = -g->ds / (sa * g->dy);
+            begin = cy - cx * step - cs * chstep;
+        }
+        for (unsigned int s = 0; s < g->ns; ++s) {
+            bo[0] = begin;
+            bo[1] = step;
+            begin += chstep;
+            bo += 2;
+        }
+        angle += g->orbit;
+    }
+    return CTR_OK;
+}
+
+// FP transpose tables (src/fp_par_2d.cc:285-330); no angle wrap, as in the
+// reference.  pos is [na][pos_stride][2].
+int ctr_fp_grad_tables(const ctr_geom* g, double* weights, double* pos, int* usex,
+                       unsigned int pos_stride)
+{
+    CTR_REQUIRE(g && weights && pos && usex, CTR_ERR_ARG, "ctr_fp_grad_tables: null argument");
+    const unsigned int need = g->nx > g->ny ? g->nx : g->ny;
+    CTR_REQUIRE(pos_stride >= need || (pos_stride == g->nx && g->nx == g->ny), CTR_ERR_ARG,
+                "ctr_fp_grad_tables: pos_stride must be >= max(nx, ny)");
+    const double cx = cent_x(*g), cy = cent_y(*g), cs = cent_s(*g);
+    const double q = M_PI / 4;
+    double angle = g->orbit_start;
+    for (unsigned int a = 0; a < g->na; ++a) {
+        const double sa = std::sin(angle), ca = std::cos(angle);
+        const bool ybranch = (angle >= q && angle < 3 * q) || (angle >= 5 * q && angle < 7 * q);
+        usex[a] = ybranch ? 0 : 1;
+        double* p = pos + 2 * static_cast<size_t>(a) * pos_stride;
+        double o1, o2, begin;
+        unsigned int n;
+        if (!ybranch) {
+            weights[a] = std::fabs(g->dy / ca);
+            o1 = g->dx * ca / g->ds;
+            o2 = -g->dy * sa / g->ds;
+            begin = cs - cx * o1 - cy * o2;
+            n = g->ny;
+        } else {
+            weights[a] = std::fabs(g->dx / sa);
+            o1 = -g->dy * sa / g->ds;
+            o2 = g->dx * ca / g->ds;
+            begin = cs - cy * o1 - cx * o2;
+            n = g->nx;
+        }
+        for (unsigned int k = 0; k < n; ++k) {
+            p[0] = begin;
+            p[1] = o1;
+            begin += o2;
+            p += 2;
+        }
+        angle += g->orbit;
+    }
+    return CTR_OK;
+}
+
+// Ram-Lak taps filter[ns + n], n in [-ns, ns] (src/filter.cc:18-65).
+int ctr_ramp_taps(const ctr_geom* g, double* filter)
+{
+    CTR_REQUIRE(g && filter, CTR_ERR_ARG, "ctr_ramp_taps: null argument");
+    const int ns = static_cast<int>(g->ns);
+    double* h = filter + ns;
+    if (g->type == 0 || g->type == 2) {
+        const double inv_ds2 = 1.0 / (g->ds * g->ds);
+        const double inv_pi2 = 1.0 / (M_PI * M_PI);
+        for (int n = -ns; n <= ns; ++n)
+            h[n] = n == 0 ? 0.25 * inv_ds2
+                          : (n % 2 == 0 ? 0.0 : -inv_pi2 * inv_ds2 / static_cast<double>(n * n));
+    } else if (g->type == 1) {
+        const double c = 1.0 / (M_PI * M_PI * g->dsd * g->dsd);
+        for (int n = -ns; n <= ns; ++n) {
+            if (n == 0) {
+                h[n] = 0.25 / (g->ds * g->ds);
+            } else if (n % 2 == 0) {
+                h[n] = 0.0;
+            } else {
+                const double sa = std::sin(static_cast<double>(n) * g->ds / g->dsd);
+                h[n] = -c / (sa * sa);
+            }
+        }
+    } else {
+        ctr::set_error("ctr_ramp_taps: unknown filter type (0 par, 1 fan, 2 flat)");
+        return CTR_ERR_ARG;
+    }
+    return CTR_OK;
+}
+
+int ctr_device_info(int* cu_count, int* clock_khz, int* lds_bytes_per_cu, char* arch, int arch_len)
+{
+    int dev = 0;
+    CTR_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    CTR_HIP(hipGetDeviceProperties(&p, dev));
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (clock_khz) *clock_khz = p.clockRate;
+    if (lds_bytes_per_cu) *lds_bytes_per_cu = static_cast<int>(p.maxSharedMemoryPerMultiProcessor);
+    if (arch && arch_len > 0) {
+        std::strncpy(arch, p.gcnArchName, arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    return CTR_OK;
+}
+
+}  // extern "C"

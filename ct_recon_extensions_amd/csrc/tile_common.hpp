@@ -1,0 +1,79 @@
+// Pixel tiling shared by the pixel-driven BP kernels and their transposes,
+// and the fixed-point scale used by the deterministic transposes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace ctr {
+
+constexpr int kAnchorChunk = 64;  // anchors computed per batch (one per lane of wave 0)
+
+template <int TW, int TH>
+struct TileShape {
+    static constexpr int kPix = TW * TH;
+    static constexpr int kBlock = kPix >= 256 ? 256 : kPix;
+    static constexpr int kPPT = kPix / kBlock;
+    static_assert(kPix % kBlock == 0, "tile must be a multiple of the block");
+};
+
+// Origin of a tile and the pixel (ic, jc) its anchor is evaluated at.
+struct TileFrame {
+    int x0, y0, ic, jc;
+    __device__ TileFrame(int tile, int tiles_x, int nx, int ny, int TW, int TH)
+    {
+        x0 = (tile % tiles_x) * TW;
+        y0 = (tile / tiles_x) * TH;
+        ic = min(x0 + TW / 2, nx - 1);
+        jc = min(y0 + TH / 2, ny - 1);
+    }
+};
+
+// Reference edge rule (src/bp_fan_2d.cc:72, src/bp_par_2d.cc:71): a sample
+// at s = s_int + frac counts iff 0 <= s <= ns-1.
+__device__ __forceinline__ bool on_detector(int s_int, float frac, int ns)
+{
+    return s_int >= 0 && (s_int < ns - 1 || (s_int == ns - 1 && frac == 0.0f));
+}
+
+// Deterministic transposes accumulate in 64-bit fixed point with a scale
+// 2^E chosen so that every channel sum stays below 2^62: stage 1 writes
+// per-block maxima of |x|, stage 2 (one block of 64) reduces them and picks
+// E.  No memset and no atomics, so the pair may sit inside a captured graph.
+constexpr int kMaxBlocks = 256;
+
+static __global__ void abs_max_partial_kernel(const float* __restrict__ x, size_t n,
+                                              float* partial)
+{
+    __shared__ float red[4];
+    float m = 0.0f;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        m = fmaxf(m, fabsf(x[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+static __global__ void fix_scale_kernel(const float* __restrict__ partial, int nblocks,
+                                        double bound_factor, double* scale)
+{
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < nblocks; i += 64) m = fmaxf(m, partial[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (threadIdx.x == 0) {
+        const double bound = fmax(static_cast<double>(m) * bound_factor, 1e-300);
+        *scale = ldexp(1.0, 62 - static_cast<int>(ceil(log2(bound))));
+    }
+}
+
+static __global__ void scale_f64_kernel(double* x, size_t n, double s)
+{
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        x[i] *= s;
+}
+
+}  // namespace ctr

@@ -1,0 +1,157 @@
+/*
+ * ctr_api.h -- C ABI of the MI355X (gfx950) 2-D CT reconstruction library.
+ *
+ * Plain pointers and sizes only.  Every operator entry point enqueues HIP
+ * work on the caller's stream and returns without synchronising; every
+ * function returns CTR_OK (0) or an error code, never throws, and leaves a
+ * message for ctr_last_error() on the calling thread.  Device pointers are
+ * HIP device (or managed) memory; host pointers are ordinary memory.
+ *
+ * Layouts follow the reference (ltldlzltl/CT_RECON_EXTENSIONS):
+ *   sinogram  proj[b][ia][is]   (view-major, channel fastest;
+ *                                src/bp_fan_2d.cc:76-77)
+ *   image     img[b][iy][ix]    (row 0 is +y; src/bp_fan_2d.cc:22-42)
+ * with b the slice of a batch of nbatch slices sharing one geometry, i.e.
+ * the NHWC [N,H,W,1] tensors of the reference's TF ops
+ * (tensorflow/bp_fan_2d_ops.cu:178-205).
+ *
+ * Each operator replaces one reference routine; the citation is on its
+ * declaration.  The reference's own FFI for this path is the numpy C ABI in
+ * include/numpy_ctext.h (re-exported unchanged) and the class API in
+ * include/bp_fan_2d.h and siblings (re-declared with hipStream_t); both are
+ * thin layers over the functions below.
+ */
+#ifndef CTR_API_H_
+#define CTR_API_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ctr_status {
+    CTR_OK = 0,
+    CTR_ERR_ARG = 1,          /* null pointer, bad enum, non-positive size  */
+    CTR_ERR_SHAPE = 2,        /* geometry the kernels cannot handle         */
+    CTR_ERR_HIP = 3,          /* a HIP runtime call failed                   */
+    CTR_ERR_UNSUPPORTED = 4,  /* a mode the library does not provide         */
+    CTR_ERR_HANDLE = 5,       /* unknown or destroyed handle                 */
+    CTR_ERR_ALLOC = 6         /* device allocation failed                    */
+};
+
+enum ctr_dtype { CTR_F32 = 0, CTR_F64 = 1 };
+
+/* One geometry record for every operator: the union of the reference's
+ * Param structs (include/bp_fan_2d.h:20-59, include/bp_fan_2d_angle.h:20-55,
+ * include/bp_par_2d.h:20-55, include/fp_par_2d.h:20-55,
+ * include/filter.h:21-36, include/fan_weighting.h:21-39).  Fields a given
+ * operator does not use are ignored.  type: ramp filter 0 par / 1 fan
+ * (equiangular) / 2 flat; fan weighting 1 fan / 2 flat.  nrow: rows of the
+ * ramp filter and fan weighting (views per slice). */
+typedef struct ctr_geom {
+    unsigned int ns, na;
+    double ds, orbit, offset_s, orbit_start;
+    unsigned int nx, ny;
+    double dx, dy, offset_x, offset_y;
+    double dso, dsd, fov;
+    int type;
+    unsigned int nrow;
+} ctr_geom;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* ctr_last_error(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int ctr_version(void);
+/* Device properties the kernels are tuned on (for reports). */
+int ctr_device_info(int* cu_count, int* clock_khz, int* lds_bytes_per_cu, char* arch,
+                    int arch_len);
+
+/* ---------------------------------------------------------------------
+ * FP64 geometry tables, built on the host in the reference's layout and
+ * accumulation order (these are the only FP64 step of the path).
+ * ------------------------------------------------------------------- */
+/* FanBackprojection2DPixDrivenPrep (src/bp_fan_2d.cc:18-44): xpos[nx],
+ * ypos[ny], sincostbl[2na] = (sin, cos) of the accumulated angle. */
+int ctr_fan_tables(const ctr_geom* g, double* xpos, double* ypos, double* sincostbl);
+/* FanBackprojection2DAnglePixDrivenPrep (src/bp_fan_2d_angle.cc:18-45). */
+int ctr_fan_angle_tables(const ctr_geom* g, const double* angles, double* xpos, double* ypos,
+                         double* sincostbl);
+/* ParallelBackprojection2DPixDrivenPrep (src/bp_par_2d.cc:18-49):
+ * xcos[ix*na + ia], ysin[iy*na + ia]. */
+int ctr_par_tables(const ctr_geom* g, double* xcos, double* ysin);
+/* ParallelProjection2DRayDrivenPrepare (src/fp_par_2d.cc:149-208):
+ * sincostbl[2na], beginoffset[na][ns][2], usex[na]. */
+int ctr_fp_tables(const ctr_geom* g, double* sincostbl, double* beginoffset, int* usex);
+/* ParallelProjection2DRayDrivenGradPrep (src/fp_par_2d.cc:285-330):
+ * weights[na], pos[na][pos_stride][2], usex[na].  The reference uses
+ * pos_stride == nx (and so requires nx == ny); pos_stride >= max(nx, ny)
+ * lifts that restriction. */
+int ctr_fp_grad_tables(const ctr_geom* g, double* weights, double* pos, int* usex,
+                       unsigned int pos_stride);
+/* RampFilterPrep (src/filter.cc:18-65): filter[2ns+1], tap n at [ns+n]. */
+int ctr_ramp_taps(const ctr_geom* g, double* filter);
+
+/* ---------------------------------------------------------------------
+ * Operators on device memory (asynchronous on `stream`, a hipStream_t;
+ * NULL is the null stream).  dtype selects float32 or float64 buffers;
+ * tables are the device copies of the FP64 tables above.
+ * ------------------------------------------------------------------- */
+/* Pixel-driven fan-beam BP, FanBackprojection2DPixDriven<T>
+ * (src/bp_fan_2d.cc:46-91) with scale = |orbit|/round(|na orbit|/pi), and
+ * FanBackprojection2DAnglePixDriven<T> (src/bp_fan_2d_angle.cc:47-91) with
+ * scale = 1.  proj [nbatch][na][ns] -> img [nbatch][ny][nx]. */
+int ctr_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+               const double* xpos, const double* ypos, const double* sincostbl, double scale,
+               void* stream);
+/* Exact transpose of ctr_fan_bp: img [nbatch][ny][nx] -> proj
+ * [nbatch][na][ns] (replaces FanBackprojection2DPixDrivenGrad<T>,
+ * src/bp_fan_2d.cc:124-170, and the broken bp_fan_2d_angle transpose).
+ * View-parallel scatter; deterministic != 0 accumulates in 64-bit fixed
+ * point so the result does not depend on the order of the contributions. */
+int ctr_fan_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
+                 const double* xpos, const double* ypos, const double* sincostbl, double scale,
+                 int deterministic, void* stream);
+/* Pixel-driven parallel BP, ParallelBackprojection2DPixDriven<T>
+ * (src/bp_par_2d.cc:51-90): img = orbit * sum_views lerp(proj, s). */
+int ctr_par_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+               const double* xcos, const double* ysin, void* stream);
+/* Exact transpose of ctr_par_bp (replaces the reference's broken
+ * ParallelBackprojection2DPixDrivenGrad<T>, src/bp_par_2d.cc:147-199). */
+int ctr_par_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
+                 const double* xcos, const double* ysin, int deterministic, void* stream);
+/* Ray-driven parallel FP, ParallelProjection2DRayDriven<T>
+ * (src/fp_par_2d.cc:210-280): img [nbatch][ny][nx] -> proj [nbatch][na][ns]. */
+int ctr_fp(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
+           const double* sincostbl, const double* beginoffset, const int* usex, void* stream);
+/* Gather-form transpose of ctr_fp, ParallelProjection2DRayDrivenGrad<T>
+ * (src/fp_par_2d.cc:332-391), channel range clamped to the detector. */
+int ctr_fp_t(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+             const double* weights, const double* pos, const int* usex, unsigned int pos_stride,
+             void* stream);
+/* Ramp filter, RampFilter<T> / RampFilterGrad<T> (src/filter.cc:67-124):
+ * out[r][is] = ds * sum_k in[r][k] h[is-k] over nrow_total rows of ns.
+ * filter: device copy of the 2ns+1 FP64 taps.  Optional fused fan weighting
+ * of the input (weight_type 1 fan / 2 flat, with g->dso, g->dsd,
+ * g->offset_s; 0 = none), FanWeighting<T> (src/fan_weighting.cc:19-35). */
+int ctr_ramp(const ctr_geom* g, int dtype, const void* in, void* out, unsigned int nrow_total,
+             const double* filter, int weight_type, void* stream);
+/* Fan / flat cosine weighting and its transpose (the same diagonal),
+ * FanWeighting<T> / FanWeightingGrad<T> (src/fan_weighting.cc:19-59). */
+int ctr_fan_weighting(const ctr_geom* g, int dtype, const void* in, void* out,
+                      unsigned int nrow_total, void* stream);
+
+/* ---------------------------------------------------------------------
+ * The reference's execution structure on this GPU, kept as a measured
+ * baseline: one launch per slice, one thread per output, FP64 tables read
+ * from global memory, FP64 arithmetic and accumulation, the stream
+ * synchronised after every launch (tensorflow/bp_fan_2d_ops.h:107-128,
+ * cuda/bp_fan_2d.cu:72-110,219-245), with a full grid.
+ * ------------------------------------------------------------------- */
+int ctr_plain_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                     const double* xpos, const double* ypos, const double* sincostbl,
+                     double scale, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTR_API_H_ */
