@@ -1,0 +1,326 @@
+// Pixel-driven backprojection engine shared by the fan- and parallel-beam
+// operators (and the ragged batch): forward kernel and view-parallel scatter
+// transpose, parameterised by a geometry policy Geo that supplies
+//   Geo::Problem, Geo::Tables, Geo::Anchor (16-byte multiple, field `is0`
+//   = first staged channel, `inside` = tile strictly inside the detector),
+//   Geo::anchor(P, T, frame, view, ilo, ihi, jlo, jhi)   FP64 -> FP32 anchor
+//   Geo::pixel(P, A, fi, fj, f, w)   channel offset f (from is0) and weight.
+//
+// Forward layout choices (gfx950):
+//  * one workgroup = one pixel tile x NB slices of a batch that share the
+//    geometry; the geometry of a pixel-view is evaluated once for all NB;
+//  * per chunk of views the detector span each tile touches is staged into
+//    LDS as spans[view][slice][LROW channels] straight from global memory
+//    with buffer loads that write LDS directly (no VGPRs, no per-row wait;
+//    the buffer range check returns 0 for channels off the detector, which
+//    is the zero padding the edge rule needs), double-buffered so the next
+//    chunk's loads are in flight while the current chunk is computed;
+//  * a pixel reads its two taps for two slices at a time with
+//    ds_read2st64_b32 (rows are LROW = 64k words apart), and a tap pair
+//    contributes w0*p[idx] + w1*p[idx+1] per slice (w0 = w(1-frac),
+//    w1 = w frac, shared by the slices): two FMAs;
+//  * sums are two-level FP32: within a chunk of 64 views, then across
+//    chunks (on the fan C2 data this adds ~1e-7 relative error, against
+//    ~9e-8 for FP64 carries, and frees the registers for a fourth wave).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "tile_common.hpp"
+
+namespace ctr {
+
+__device__ __forceinline__ float uniform_f(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ int uniform_i(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+template <class Geo>
+struct BpLaunch {
+    typename Geo::Problem P;
+    int nbatch, tiles_x, tiles_y;
+    int lmax;    // channels a (view, slice) span needs (<= LROW)
+    int vchunk;  // views per LDS buffer
+    double scale;
+    const float* proj;
+    float* img;
+};
+
+// Issue the DMA loads of the spans of views [v, v+nv) into buf:
+// row (t, b) <- proj[b0+b][v+t][is0(t) .. is0(t)+LROW).  Each wave stages
+// whole views: one anchor read per view, then NB row loads whose descriptors
+// differ by a uniform stride.
+template <class Geo, int NB, int LROW, int NWAVES>
+__device__ __forceinline__ void stage_spans(const BpLaunch<Geo>& L,
+                                            const typename Geo::Anchor* anchors, float* buf,
+                                            int v, int vrel, int nv, int b0, int nb, int wave,
+                                            int lane)
+{
+    const auto& P = L.P;
+    const unsigned int row_bytes = static_cast<unsigned int>(P.ns) * 4u;
+    const size_t slice_stride = static_cast<size_t>(P.na) * P.ns;
+    for (int t = wave; t < nv; t += NWAVES) {
+        const int is0 = uniform_i(anchors[vrel + t].is0);
+        const int voff = (is0 + lane) * 4;
+        const float* src = L.proj + (static_cast<size_t>(b0) * P.na + v + t) * P.ns;
+        float* dst = buf + t * NB * LROW;
+#pragma unroll 2
+        for (int b = 0; b < NB; ++b) {
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<float*>(src), 0, b < nb ? row_bytes : 0u, 0x00020000);
+#pragma unroll
+            for (int c = 0; c < LROW; c += 64)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs, (__attribute__((address_space(3))) void*)(dst + c), 4, voff + c * 4, 0,
+                    0, 0);
+            src += slice_stride;
+            dst += LROW;
+        }
+    }
+}
+
+// Geometry of one pixel-view in the form the accumulation needs: LDS word
+// index of the left tap and the two tap weights.
+template <class Geo, int LROW>
+__device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
+                                            const typename Geo::Anchor& A, int inside, int is0,
+                                            float fi, float fj, int& idx, float& w0, float& w1)
+{
+    float f, w;
+    Geo::pixel(P, A, fi, fj, f, w);
+    const float fl = floorf(f);
+    const float fr = f - fl;
+    idx = static_cast<int>(fl);
+    if (__builtin_expect(!inside, 0)) w = on_detector(is0 + idx, fr, P.ns) ? w : 0.0f;
+    idx = min(max(idx, 0), LROW - 2);
+    w1 = w * fr;
+    w0 = w - w1;
+}
+
+template <class Geo, int TW, int TH, int NB, int LROW>
+__global__ __launch_bounds__((TileShape<TW, TH>::kBlock))
+__attribute__((amdgpu_waves_per_eu(4, 4))) void bp_forward_kernel(
+    const typename Geo::Tables T, const BpLaunch<Geo> L)
+{
+    using TS = TileShape<TW, TH>;
+    using Anchor = typename Geo::Anchor;
+    constexpr int BLOCK = TS::kBlock, PPT = TS::kPPT, NWAVES = BLOCK / 64;
+    static_assert(LROW % 64 == 0, "span rows are whole waves of dwords");
+    extern __shared__ __align__(16) unsigned char smem[];
+    Anchor* anchors = reinterpret_cast<Anchor*>(smem);
+    float* spans = reinterpret_cast<float*>(smem + kAnchorChunk * sizeof(Anchor));
+    const int bufsz = L.vchunk * NB * LROW;  // floats per buffer
+
+    const auto& P = L.P;
+    const int tid = threadIdx.x, lane = tid & 63, wave = uniform_i(tid >> 6);
+    const TileFrame F(blockIdx.x, L.tiles_x, P.nx, P.ny, TW, TH);
+    const int b0 = blockIdx.y * NB;
+    const int nb = min(NB, L.nbatch - b0);
+    const int ilo = F.x0 - F.ic, ihi = F.x0 + TW - 1 - F.ic;
+    const int jlo = F.y0 - F.jc, jhi = F.y0 + TH - 1 - F.jc;
+
+    float fi[PPT], fj[PPT];
+    float total[PPT][NB];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int p = tid + k * BLOCK;
+        fi[k] = static_cast<float>(F.x0 + p % TW - F.ic);
+        fj[k] = static_cast<float>(F.y0 + p / TW - F.jc);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) total[k][b] = 0.0f;
+    }
+
+    for (int va = 0; va < P.na; va += kAnchorChunk) {
+        const int nva = min(kAnchorChunk, P.na - va);
+        __syncthreads();  // previous anchors and spans are no longer read
+        if (tid < nva) anchors[tid] = Geo::anchor(P, T, F, va + tid, ilo, ihi, jlo, jhi);
+        __syncthreads();
+        stage_spans<Geo, NB, LROW, NWAVES>(L, anchors, spans, va, 0, min(L.vchunk, nva), b0, nb,
+                                           wave, lane);
+        float acc[PPT][NB];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) acc[k][b] = 0.0f;
+
+        int cur = 0;
+        for (int vc = 0; vc < nva; vc += L.vchunk) {
+            const int nv = min(L.vchunk, nva - vc);
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's span loads have landed
+            __syncthreads();                // ... and every other wave's
+            // prefetch the next chunk into the other buffer
+            if (vc + L.vchunk < nva)
+                stage_spans<Geo, NB, LROW, NWAVES>(L, anchors, spans + (cur ^ 1) * bufsz,
+                                                   va + vc + L.vchunk, vc + L.vchunk,
+                                                   min(L.vchunk, nva - vc - L.vchunk), b0, nb,
+                                                   wave, lane);
+            const float* sbuf = spans + cur * bufsz;
+            // Software pipeline over the views of the chunk: the taps of view
+            // t are read while the geometry of view t+1 is computed.
+            int idx[PPT];
+            float w0[PPT], w1[PPT];
+            {
+                const Anchor A = anchors[vc];
+                const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
+#pragma unroll
+                for (int k = 0; k < PPT; ++k)
+                    tap_weights<Geo, LROW>(P, A, inside, is0, fi[k], fj[k], idx[k], w0[k], w1[k]);
+            }
+            for (int t = 0; t < nv; ++t) {
+                const float* sview = sbuf + t * NB * LROW;
+                float p0[PPT][NB], p1[PPT][NB];
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    // Right taps come through a second base whose +1 the
+                    // compiler cannot see, so the loads pair up per tap across
+                    // slices (rows LROW words apart: ds_read2st64_b32, no
+                    // per-slice address arithmetic).
+                    int one = 1;
+                    asm volatile("" : "+v"(one));
+                    const float* q0 = sview + idx[k];
+                    const float* q1 = sview + idx[k] + one;
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        p0[k][b] = q0[b * LROW];
+                        p1[k][b] = q1[b * LROW];
+                    }
+                }
+                float cw0[PPT], cw1[PPT];
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    cw0[k] = w0[k];
+                    cw1[k] = w1[k];
+                }
+                if (t + 1 < nv) {
+                    const Anchor A = anchors[vc + t + 1];
+                    const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
+#pragma unroll
+                    for (int k = 0; k < PPT; ++k)
+                        tap_weights<Geo, LROW>(P, A, inside, is0, fi[k], fj[k], idx[k], w0[k],
+                                               w1[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < PPT; ++k)
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+                        acc[k][b] = fmaf(cw1[k], p1[k][b], fmaf(cw0[k], p0[k][b], acc[k][b]));
+            }
+            cur ^= 1;
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) total[k][b] += acc[k][b];
+    }
+
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int p = tid + k * BLOCK;
+        const int ix = F.x0 + p % TW, iy = F.y0 + p / TW;
+        if (ix < P.nx && iy < P.ny) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+                if (b < nb)
+                    L.img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix] =
+                        static_cast<float>(static_cast<double>(total[k][b]) * L.scale);
+        }
+    }
+}
+
+// View-parallel scatter transpose.  One workgroup owns the detector rows of
+// one view for NB slices in LDS, walks every tile with the forward kernel's
+// tiling and anchors (bit-identical weights), scatters with LDS atomics and
+// writes the rows once.  DET: 64-bit fixed-point accumulation (integer
+// atomics are associative, so the result is independent of ordering).
+template <class Geo>
+struct BptLaunch {
+    typename Geo::Problem P;
+    int nbatch, tiles_x, tiles_y;
+    double scale;
+    const double* fix_scale;  // DET: device pointer to 2^E
+};
+
+template <class Geo, int TW, int TH, int NB, bool DET>
+__global__ __launch_bounds__(256) void bp_transpose_kernel(const float* __restrict__ img,
+                                                           float* __restrict__ proj,
+                                                           const typename Geo::Tables T,
+                                                           const BptLaunch<Geo> L)
+{
+    using TS = TileShape<TW, TH>;
+    using Anchor = typename Geo::Anchor;
+    constexpr int BLOCK = TS::kBlock, PPT = TS::kPPT;
+    static_assert(BLOCK == 256, "transpose assumes 256-thread tiles");
+    using Acc = typename std::conditional<DET, unsigned long long, float>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    Anchor* anchors = reinterpret_cast<Anchor*>(smem);
+    Acc* rows = reinterpret_cast<Acc*>(smem + kAnchorChunk * sizeof(Anchor));
+    const auto& P = L.P;
+    const int tid = threadIdx.x;
+    const int view = blockIdx.x;
+    const int b0 = blockIdx.y * NB;
+    const int nb = min(NB, L.nbatch - b0);
+    const int rlen = P.ns + 1;  // pad slot for the right tap at s == ns-1
+    double fixs = 1.0;
+    if constexpr (DET) fixs = *L.fix_scale;
+    for (int e = tid; e < NB * rlen; e += BLOCK) rows[e] = Acc(0);
+
+    const int ntiles = L.tiles_x * L.tiles_y;
+    for (int tc = 0; tc < ntiles; tc += kAnchorChunk) {
+        const int ntc = min(kAnchorChunk, ntiles - tc);
+        __syncthreads();
+        if (tid < ntc) {
+            const TileFrame F(tc + tid, L.tiles_x, P.nx, P.ny, TW, TH);
+            anchors[tid] = Geo::anchor(P, T, F, view, F.x0 - F.ic, F.x0 + TW - 1 - F.ic,
+                                       F.y0 - F.jc, F.y0 + TH - 1 - F.jc);
+        }
+        __syncthreads();
+        for (int t = 0; t < ntc; ++t) {
+            const Anchor A = anchors[t];
+            const TileFrame F(tc + t, L.tiles_x, P.nx, P.ny, TW, TH);
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                const int p = tid + k * BLOCK;
+                const int ix = F.x0 + p % TW, iy = F.y0 + p / TW;
+                if (ix >= P.nx || iy >= P.ny) continue;
+                float f, w;
+                Geo::pixel(P, A, static_cast<float>(ix - F.ic), static_cast<float>(iy - F.jc), f,
+                           w);
+                const float fl = floorf(f);
+                const float fr = f - fl;
+                const int s = A.is0 + static_cast<int>(fl);
+                if (!on_detector(s, fr, P.ns)) continue;
+                const float w1 = w * fr, w0 = w - w1;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    if (b >= nb) break;
+                    const float v = img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix];
+                    if constexpr (DET) {
+                        atomicAdd(&rows[b * rlen + s], static_cast<unsigned long long>(
+                                      __double2ll_rn(static_cast<double>(w0 * v) * fixs)));
+                        atomicAdd(&rows[b * rlen + s + 1], static_cast<unsigned long long>(
+                                      __double2ll_rn(static_cast<double>(w1 * v) * fixs)));
+                    } else {
+                        atomicAdd(&rows[b * rlen + s], w0 * v);
+                        atomicAdd(&rows[b * rlen + s + 1], w1 * v);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nb * P.ns; e += BLOCK) {
+        const int b = e / P.ns, s = e - b * P.ns;
+        double v;
+        if constexpr (DET)
+            v = static_cast<double>(static_cast<long long>(rows[b * rlen + s])) / fixs;
+        else
+            v = rows[b * rlen + s];
+        proj[(static_cast<size_t>(b0 + b) * P.na + view) * P.ns + s] =
+            static_cast<float>(v * L.scale);
+    }
+}
+
+}  // namespace ctr

@@ -1,28 +1,16 @@
-// Fan-beam pixel-driven backprojection for gfx950, its exact transpose, and
-// the reference-structured baseline.
-//
-// Forward (fan_bp_kernel): one workgroup per (pixel tile, group of NB
-// slices).  All slices of a batch share the geometry, so the per
-// pixel-view geometry (channel position and weight, ~20 VALU ops) is
-// evaluated once and reused for NB slices, whose cost is then an LDS read
-// pair, a lerp and an FMA.  Per chunk of views the detector span each tile
-// touches is staged into LDS for every slice (zero outside [0, ns-1]);
-// per-(tile, view) anchors are computed in FP64 inside the kernel
-// (fan_geom.hpp).  Partial sums are kept in FP32 within a chunk of views and
-// carried across chunks in FP64.
-//
-// Transpose (fan_bpt_kernel): view-parallel scatter.  One workgroup per
-// (view, slice group) owns that view's detector rows in LDS, walks all
-// pixel tiles with the same tiling and anchors as the forward kernel (so the
-// weights are bit-identical), scatters with LDS atomics and writes the rows
-// once, coalesced: no global atomics and no zero-fill pass.  Deterministic
-// mode accumulates in 64-bit fixed point (integer LDS atomics), which is
-// associative, so the result is independent of the order of contributions.
+// Fan-beam pixel-driven backprojection for gfx950 (bp_fan_2d and
+// bp_fan_2d_angle), its exact transpose, and the reference-structured
+// baseline.  The forward kernel and the transpose are the BP engine
+// (bp_engine.hpp) with the fan geometry policy (fan_geom.hpp): workgroups of
+// one pixel tile x NB slices sharing the geometry, LDS-staged detector spans,
+// FP64 per-(tile, view) anchors and FP32 pixel offsets; the transpose is a
+// view-parallel scatter into LDS rows with an order-independent fixed-point
+// mode.
 //
 // Edge rule (src/bp_fan_2d.cc:72-80): a pixel-view contributes iff
-// 0 <= s <= ns-1, with taps floor(s) and ceil(s).  With s = is0 + f the
-// test is  s_int >= 0  and  (s_int < ns-1  or  (s_int == ns-1 and frac == 0)).
+// 0 <= s <= ns-1, with taps floor(s) and ceil(s).
 #include "ctr_common.hpp"
+#include "bp_engine.hpp"
 #include "fan_geom.hpp"
 #include "tile_common.hpp"
 
@@ -31,212 +19,6 @@
 #include <type_traits>
 
 namespace ctr {
-
-struct FanLaunch {
-    FanProblem P;
-    int nbatch;
-    int tiles_x, tiles_y;
-    int lmax;      // staged channels per (view, slice)
-    int vchunk;    // views staged per LDS round
-    double scale;  // output scale
-};
-
-template <int TW, int TH, int NB, bool EXACT>
-__global__ __launch_bounds__((TileShape<TW, TH>::kBlock)) void fan_bp_kernel(
-    const float* __restrict__ proj, float* __restrict__ img, const double* __restrict__ xpos,
-    const double* __restrict__ ypos, const double* __restrict__ sincos, const FanLaunch L)
-{
-    using TS = TileShape<TW, TH>;
-    constexpr int BLOCK = TS::kBlock, PPT = TS::kPPT;
-    extern __shared__ __align__(16) unsigned char smem[];
-    FanAnchor* anchors = reinterpret_cast<FanAnchor*>(smem);
-    float* spans = reinterpret_cast<float*>(smem + kAnchorChunk * sizeof(FanAnchor));
-
-    const FanProblem& P = L.P;
-    const int tid = threadIdx.x;
-    const TileFrame F(blockIdx.x, L.tiles_x, P.nx, P.ny, TW, TH);
-    const int b0 = blockIdx.y * NB;
-    const int nb = min(NB, L.nbatch - b0);
-    const int lmax = L.lmax;
-    const double xc = xpos[F.ic], yc = ypos[F.jc];
-
-    float fi[PPT], fj[PPT];
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        const int p = tid + k * BLOCK;
-        fi[k] = static_cast<float>(F.x0 + p % TW - F.ic);
-        fj[k] = static_cast<float>(F.y0 + p / TW - F.jc);
-    }
-    double acc64[PPT][NB];
-#pragma unroll
-    for (int k = 0; k < PPT; ++k)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) acc64[k][b] = 0.0;
-
-    const int ilo = F.x0 - F.ic, ihi = F.x0 + TW - 1 - F.ic;
-    const int jlo = F.y0 - F.jc, jhi = F.y0 + TH - 1 - F.jc;
-
-    for (int va = 0; va < P.na; va += kAnchorChunk) {
-        const int nva = min(kAnchorChunk, P.na - va);
-        __syncthreads();  // previous anchors and spans are no longer read
-        if (tid < nva)
-            anchors[tid] = fan_anchor(P, xc, yc, sincos[2 * (va + tid)],
-                                      sincos[2 * (va + tid) + 1], ilo, ihi, jlo, jhi);
-        float acc[PPT][NB];
-#pragma unroll
-        for (int k = 0; k < PPT; ++k)
-#pragma unroll
-            for (int b = 0; b < NB; ++b) acc[k][b] = 0.0f;
-
-        for (int vc = 0; vc < nva; vc += L.vchunk) {
-            const int nv = min(L.vchunk, nva - vc);
-            __syncthreads();  // anchors written / previous spans consumed
-            // Stage spans: row (t, b) holds channels is0 .. is0+lmax-1.
-            const int rows = nv * NB;
-            for (int e = tid; e < rows * lmax; e += BLOCK) {
-                const int row = e / lmax, c = e - row * lmax;
-                const int t = row / NB, b = row - t * NB;
-                const int ch = anchors[vc + t].is0 + c;
-                float val = 0.0f;
-                if (b < nb && ch >= 0 && ch < P.ns)
-                    val = proj[(static_cast<size_t>(b0 + b) * P.na + va + vc + t) * P.ns + ch];
-                spans[e] = val;
-            }
-            __syncthreads();
-            for (int t = 0; t < nv; ++t) {
-                const FanAnchor A = anchors[vc + t];
-                const float* srow = spans + t * NB * lmax;
-#pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    float f, w;
-                    fan_pixel<EXACT>(P, A, fi[k], fj[k], f, w);
-                    const float fl = floorf(f);
-                    const float fr = f - fl;
-                    int idx = static_cast<int>(fl);
-                    if (!A.inside) {
-                        const int s = A.is0 + idx;
-                        const bool ok = on_detector(s, fr, P.ns);
-                        w = ok ? w : 0.0f;
-                    }
-                    idx = min(max(idx, 0), lmax - 2);
-#pragma unroll
-                    for (int b = 0; b < NB; ++b) {
-                        const float p0 = srow[b * lmax + idx];
-                        const float p1 = srow[b * lmax + idx + 1];
-                        acc[k][b] = fmaf(w, fmaf(fr, p1 - p0, p0), acc[k][b]);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < PPT; ++k)
-#pragma unroll
-            for (int b = 0; b < NB; ++b) acc64[k][b] += static_cast<double>(acc[k][b]);
-    }
-
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        const int p = tid + k * BLOCK;
-        const int ix = F.x0 + p % TW, iy = F.y0 + p / TW;
-        if (ix < P.nx && iy < P.ny) {
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-                if (b < nb)
-                    img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix] =
-                        static_cast<float>(acc64[k][b] * L.scale);
-        }
-    }
-}
-
-// ---------------------------------------------------------------- transpose
-struct FanTLaunch {
-    FanProblem P;
-    int nbatch;
-    int tiles_x, tiles_y;
-    double scale;
-    const double* fix_scale;  // deterministic mode: device pointer to 2^E
-};
-
-template <int TW, int TH, int NB, bool EXACT, bool DET>
-__global__ __launch_bounds__(256) void fan_bpt_kernel(
-    const float* __restrict__ img, float* __restrict__ proj, const double* __restrict__ xpos,
-    const double* __restrict__ ypos, const double* __restrict__ sincos, const FanTLaunch L)
-{
-    using TS = TileShape<TW, TH>;
-    constexpr int BLOCK = TS::kBlock, PPT = TS::kPPT;
-    static_assert(BLOCK == 256, "transpose assumes 256-thread tiles");
-    using Acc = typename std::conditional<DET, unsigned long long, float>::type;
-    extern __shared__ __align__(16) unsigned char smem[];
-    FanAnchor* anchors = reinterpret_cast<FanAnchor*>(smem);
-    Acc* rows = reinterpret_cast<Acc*>(smem + kAnchorChunk * sizeof(FanAnchor));
-
-    const FanProblem& P = L.P;
-    const int tid = threadIdx.x;
-    const int view = blockIdx.x;
-    const int b0 = blockIdx.y * NB;
-    const int nb = min(NB, L.nbatch - b0);
-    const int rlen = P.ns + 1;  // one pad slot for the ceil tap at s == ns-1
-    const double sb = sincos[2 * view], cb = sincos[2 * view + 1];
-    double fixs = 1.0;
-    if constexpr (DET) fixs = *L.fix_scale;
-
-    for (int e = tid; e < NB * rlen; e += BLOCK) rows[e] = Acc(0);
-
-    const int ntiles = L.tiles_x * L.tiles_y;
-    for (int tc = 0; tc < ntiles; tc += kAnchorChunk) {
-        const int ntc = min(kAnchorChunk, ntiles - tc);
-        __syncthreads();
-        if (tid < ntc) {
-            const TileFrame F(tc + tid, L.tiles_x, P.nx, P.ny, TW, TH);
-            anchors[tid] = fan_anchor(P, xpos[F.ic], ypos[F.jc], sb, cb, F.x0 - F.ic,
-                                      F.x0 + TW - 1 - F.ic, F.y0 - F.jc, F.y0 + TH - 1 - F.jc);
-        }
-        __syncthreads();
-        for (int t = 0; t < ntc; ++t) {
-            const FanAnchor A = anchors[t];
-            const TileFrame F(tc + t, L.tiles_x, P.nx, P.ny, TW, TH);
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                const int p = tid + k * BLOCK;
-                const int ix = F.x0 + p % TW, iy = F.y0 + p / TW;
-                if (ix >= P.nx || iy >= P.ny) continue;
-                float f, w;
-                fan_pixel<EXACT>(P, A, static_cast<float>(ix - F.ic),
-                                 static_cast<float>(iy - F.jc), f, w);
-                const float fl = floorf(f);
-                const float fr = f - fl;
-                const int s = A.is0 + static_cast<int>(fl);
-                const bool ok = on_detector(s, fr, P.ns);
-                if (!ok) continue;
-                const float w1 = w * fr, w0 = w - w1;
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    if (b >= nb) break;
-                    const float v = img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix];
-                    if constexpr (DET) {
-                        const long long q0 = __double2ll_rn(static_cast<double>(w0 * v) * fixs);
-                        const long long q1 = __double2ll_rn(static_cast<double>(w1 * v) * fixs);
-                        atomicAdd(&rows[b * rlen + s], static_cast<unsigned long long>(q0));
-                        atomicAdd(&rows[b * rlen + s + 1], static_cast<unsigned long long>(q1));
-                    } else {
-                        atomicAdd(&rows[b * rlen + s], w0 * v);
-                        atomicAdd(&rows[b * rlen + s + 1], w1 * v);
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < nb * P.ns; e += BLOCK) {
-        const int b = e / P.ns, s = e - b * P.ns;
-        double v;
-        if constexpr (DET)
-            v = static_cast<double>(static_cast<long long>(rows[b * rlen + s])) / fixs;
-        else
-            v = rows[b * rlen + s];
-        proj[(static_cast<size_t>(b0 + b) * P.na + view) * P.ns + s] = static_cast<float>(v * L.scale);
-    }
-}
 
 // ------------------------------------------------- reference-structured path
 template <typename T>
@@ -300,6 +82,18 @@ __global__ void fan_bpt_f64_kernel(const double* __restrict__ img, double* __res
 // ---------------------------------------------------------------- host side
 namespace {
 
+// Smallest source-to-pixel distance over the tile-padded pixel grid (every
+// source position sits at distance dso from the rotation centre).
+double fan_min_distance(const ctr_geom& g, int TW, int TH)
+{
+    const double cx = cent_x(g), cy = cent_y(g);
+    const double wx = std::max(std::fabs(cx), std::fabs(div_up(g.nx, TW) * TW - 1 - cx)) *
+                      std::fabs(g.dx);
+    const double wy = std::max(std::fabs(cy), std::fabs(div_up(g.ny, TH) * TH - 1 - cy)) *
+                      std::fabs(g.dy);
+    return g.dso - std::sqrt(wx * wx + wy * wy);
+}
+
 // Geometry checks and the LDS span / series-accuracy bounds for a tile size.
 int fan_problem(const ctr_geom& g, int TW, int TH, FanProblem& P, int& lmax, bool& exact)
 {
@@ -323,49 +117,78 @@ int fan_problem(const ctr_geom& g, int TW, int TH, FanProblem& P, int& lmax, boo
     P.p2 = static_cast<float>(1.0 / (5 * k * k * k * k));
     P.g1 = static_cast<float>(-1.0 / (k * k));
     P.g2 = static_cast<float>(1.0 / (k * k * k * k));
-    // Farthest pixel (tile-padded grid) from the rotation centre.
-    const double cx = cent_x(g), cy = cent_y(g);
-    const double wx = std::max(std::fabs(-cx), std::fabs(div_up(g.nx, TW) * TW - 1 - cx)) *
-                      std::fabs(g.dx);
-    const double wy = std::max(std::fabs(-cy), std::fabs(div_up(g.ny, TH) * TH - 1 - cy)) *
-                      std::fabs(g.dy);
-    const double rimg = std::sqrt(wx * wx + wy * wy);
-    const double lmin = g.dso - rimg;
+    const double lmin = fan_min_distance(g, TW, TH);
     CTR_REQUIRE(lmin > 0.05 * g.dso, CTR_ERR_SHAPE,
                 "fan BP: the source comes too close to the image (dso <= image radius)");
-    // channels per pixel step <= k * pitch / distance
-    const double ext = TW * std::fabs(g.dx) + TH * std::fabs(g.dy);
-    lmax = static_cast<int>(std::ceil(k * ext / lmin)) + 4;
+    // A tile of diagonal D seen from distance >= lmin subtends at most
+    // 2 asin(D / (2 lmin)); the staged span needs that many channels plus
+    // the floor / right-tap / rounding margin.
+    const double diag = std::hypot(TW * std::fabs(g.dx), TH * std::fabs(g.dy));
+    const double ang = 2.0 * std::asin(std::min(1.0, diag / (2.0 * lmin)));
+    lmax = static_cast<int>(std::ceil(k * ang)) + 5;
     lmax = (lmax + 1) & ~1;
-    const double umax = std::tan(0.5 * ext / lmin);
+    // |u| = tan of the angle between a pixel and its anchor (at most half the
+    // tile's angle); the series drop terms of order k u^7 / 7 and u^6.
+    const double umax = std::tan(0.5 * ang);
     exact = (k * std::pow(umax, 7) / 7 > 2e-7) || (std::pow(umax, 6) > 5e-8) || umax > 0.25;
     return CTR_OK;
 }
 
-template <int TW, int TH, int NB, bool EXACT>
-int launch_fan_bp(const FanLaunch& L, const float* proj, float* img, const double* xpos,
-                  const double* ypos, const double* sc, hipStream_t st)
+template <int TW, int TH, int NB, bool EXACT, int LROW>
+int launch_fan_bp_l(BpLaunch<FanGeo<EXACT>> L, const FanTables& T, hipStream_t st)
 {
     using TS = TileShape<TW, TH>;
+    // two span buffers of vchunk x NB rows of LROW floats must fit in LDS
+    L.vchunk = std::max(1, std::min(L.vchunk, (150 * 1024 / 8) / (NB * LROW)));
     const size_t lds = kAnchorChunk * sizeof(FanAnchor) +
-                       static_cast<size_t>(L.vchunk) * NB * L.lmax * sizeof(float);
-    CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "fan BP: detector span too wide for LDS");
-    auto kern = fan_bp_kernel<TW, TH, NB, EXACT>;
+                       2 * static_cast<size_t>(L.vchunk) * NB * LROW * sizeof(float);
+    CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE,
+                "fan BP: view chunk x slices x span exceeds LDS; lower the view chunk");
+    auto kern = bp_forward_kernel<FanGeo<EXACT>, TW, TH, NB, LROW>;
     if (lds > 64 * 1024)
         CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(lds)));
     dim3 grid(L.tiles_x * L.tiles_y, div_up(L.nbatch, NB));
-    hipLaunchKernelGGL(kern, grid, dim3(TS::kBlock), lds, st, proj, img, xpos, ypos, sc, L);
-    return check_launch("fan_bp_kernel");
+    hipLaunchKernelGGL(kern, grid, dim3(TS::kBlock), lds, st, T, L);
+    return check_launch("bp_forward_kernel<fan>");
+}
+
+template <int TW, int TH, int NB, bool EXACT>
+int launch_fan_bp(const BpLaunch<FanGeo<EXACT>>& L, const FanTables& T, hipStream_t st)
+{
+    if (L.lmax <= 64) return launch_fan_bp_l<TW, TH, NB, EXACT, 64>(L, T, st);
+    if (L.lmax <= 128) return launch_fan_bp_l<TW, TH, NB, EXACT, 128>(L, T, st);
+    if (L.lmax <= 256) return launch_fan_bp_l<TW, TH, NB, EXACT, 256>(L, T, st);
+    if (L.lmax <= 512) return launch_fan_bp_l<TW, TH, NB, EXACT, 512>(L, T, st);
+    set_error("fan BP: a pixel tile spans more than 512 detector channels");
+    return CTR_ERR_SHAPE;
 }
 
 template <int TW, int TH, int NB>
-int launch_fan_bp_nb(const FanLaunch& L, bool exact, const float* proj, float* img,
-                     const double* xpos, const double* ypos, const double* sc, hipStream_t st)
+int launch_fan_bp_nb(const ctr_geom& g, const FanProblem& P, int lmax, bool exact, int nbatch,
+                     int vchunk, double scale, const float* proj, float* img, const FanTables& T,
+                     hipStream_t st)
 {
-    return exact ? launch_fan_bp<TW, TH, NB, true>(L, proj, img, xpos, ypos, sc, st)
-                 : launch_fan_bp<TW, TH, NB, false>(L, proj, img, xpos, ypos, sc, st);
+    auto fill = [&](auto& L) {
+        L.P = P;
+        L.nbatch = nbatch;
+        L.tiles_x = div_up(g.nx, TW);
+        L.tiles_y = div_up(g.ny, TH);
+        L.lmax = lmax;
+        L.vchunk = vchunk;
+        L.scale = scale;
+        L.proj = proj;
+        L.img = img;
+    };
+    if (exact) {
+        BpLaunch<FanGeo<true>> L{};
+        fill(L);
+        return launch_fan_bp<TW, TH, NB, true>(L, T, st);
+    }
+    BpLaunch<FanGeo<false>> L{};
+    fill(L);
+    return launch_fan_bp<TW, TH, NB, false>(L, T, st);
 }
 
 }  // namespace
@@ -376,66 +199,99 @@ double fan_bp_scale(const ctr_geom& g)
     return std::fabs(g.orbit) / factor;
 }
 
-// Tunables of the forward kernel (the autotuner writes them).
+// Tunables of the forward kernel (set by the autotuner through
+// ctr_fan_bp_config): tile 0 = 16x16, 1 = 8x8, 2 = 32x8; views per LDS round;
+// slices per workgroup (0 = automatic).
 struct FanBpConfig {
-    int tile = 0;    // 0: 16x16, 1: 8x8, 2: 32x8
-    int vchunk = 16;
-    int nb = 0;      // 0: automatic
+    int tile = 0;
+    int vchunk = 4;
+    int nb = 0;
 };
-static thread_local FanBpConfig g_fan_cfg;
+static FanBpConfig g_fan_cfg;
+
+void fan_bp_set_config(int tile, int vchunk, int nb)
+{
+    g_fan_cfg.tile = (tile >= 0 && tile <= 2) ? tile : 0;
+    g_fan_cfg.vchunk = (vchunk > 0 && vchunk <= kAnchorChunk) ? vchunk : 4;
+    g_fan_cfg.nb = (nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16) ? nb : 0;
+}
+
+static int auto_nb(int nbatch)
+{
+    return nbatch >= 16 ? 16 : nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
+}
 
 int fan_bp_f32(const ctr_geom& g, const float* proj, float* img, int nbatch, const double* xpos,
                const double* ypos, const double* sc, double scale, hipStream_t st)
 {
     static const int tw_of[3] = {16, 8, 32}, th_of[3] = {16, 8, 8};
-    const int tile = g_fan_cfg.tile;
-    const int TW = tw_of[tile], TH = th_of[tile];
-    FanLaunch L{};
+    FanBpConfig cfg = g_fan_cfg;
+    FanProblem P{};
+    int lmax = 0;
     bool exact = false;
-    int rc = fan_problem(g, TW, TH, L.P, L.lmax, exact);
+    int rc = fan_problem(g, tw_of[cfg.tile], th_of[cfg.tile], P, lmax, exact);
     if (rc) return rc;
-    L.nbatch = nbatch;
-    L.tiles_x = div_up(g.nx, TW);
-    L.tiles_y = div_up(g.ny, TH);
-    L.vchunk = std::max(1, std::min(g_fan_cfg.vchunk, kAnchorChunk));
-    L.scale = scale;
-    int nb = g_fan_cfg.nb;
-    if (nb == 0) nb = nbatch >= 16 ? 16 : nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
-#define CTR_FAN_NB(TW_, TH_)                                                                  \
-    switch (nb) {                                                                             \
-        case 1: return launch_fan_bp_nb<TW_, TH_, 1>(L, exact, proj, img, xpos, ypos, sc, st); \
-        case 2: return launch_fan_bp_nb<TW_, TH_, 2>(L, exact, proj, img, xpos, ypos, sc, st); \
-        case 4: return launch_fan_bp_nb<TW_, TH_, 4>(L, exact, proj, img, xpos, ypos, sc, st); \
-        case 8: return launch_fan_bp_nb<TW_, TH_, 8>(L, exact, proj, img, xpos, ypos, sc, st); \
-        default: return launch_fan_bp_nb<TW_, TH_, 16>(L, exact, proj, img, xpos, ypos, sc, st); \
+    if (lmax > 256 && cfg.tile != 1) {
+        // coarse pixels or a close source: smaller tiles keep the span short
+        cfg.tile = 1;
+        rc = fan_problem(g, 8, 8, P, lmax, exact);
+        if (rc) return rc;
     }
-    if (tile == 1) { CTR_FAN_NB(8, 8) }
-    if (tile == 2) { CTR_FAN_NB(32, 8) }
+    const FanTables T{xpos, ypos, sc};
+    const int nb = cfg.nb ? cfg.nb : auto_nb(nbatch);
+#define CTR_FAN_NB(TW_, TH_)                                                                   \
+    switch (nb) {                                                                              \
+        case 1: return launch_fan_bp_nb<TW_, TH_, 1>(g, P, lmax, exact, nbatch, cfg.vchunk,     \
+                                                     scale, proj, img, T, st);                 \
+        case 2: return launch_fan_bp_nb<TW_, TH_, 2>(g, P, lmax, exact, nbatch, cfg.vchunk,     \
+                                                     scale, proj, img, T, st);                 \
+        case 4: return launch_fan_bp_nb<TW_, TH_, 4>(g, P, lmax, exact, nbatch, cfg.vchunk,     \
+                                                     scale, proj, img, T, st);                 \
+        case 8: return launch_fan_bp_nb<TW_, TH_, 8>(g, P, lmax, exact, nbatch, cfg.vchunk,     \
+                                                     scale, proj, img, T, st);                 \
+        default: return launch_fan_bp_nb<TW_, TH_, 16>(g, P, lmax, exact, nbatch, cfg.vchunk,   \
+                                                       scale, proj, img, T, st);               \
+    }
+    if (cfg.tile == 1) { CTR_FAN_NB(8, 8) }
+    if (cfg.tile == 2) { CTR_FAN_NB(32, 8) }
     CTR_FAN_NB(16, 16)
 #undef CTR_FAN_NB
 }
 
-void fan_bp_set_config(int tile, int vchunk, int nb)
+template <bool EXACT, int NB, bool DET>
+static int launch_fan_bpt(const ctr_geom& g, const FanProblem& P, int nbatch, double scale,
+                          const double* fixs, const float* img, float* proj, const FanTables& T,
+                          hipStream_t st)
 {
-    g_fan_cfg.tile = (tile >= 0 && tile <= 2) ? tile : 0;
-    g_fan_cfg.vchunk = vchunk > 0 ? vchunk : 16;
-    g_fan_cfg.nb = nb;
+    constexpr int TW = 16, TH = 16;
+    BptLaunch<FanGeo<EXACT>> L{};
+    L.P = P;
+    L.nbatch = nbatch;
+    L.tiles_x = div_up(g.nx, TW);
+    L.tiles_y = div_up(g.ny, TH);
+    L.scale = scale;
+    L.fix_scale = fixs;
+    const size_t lds = kAnchorChunk * sizeof(FanAnchor) +
+                       static_cast<size_t>(NB) * (g.ns + 1) * (DET ? 8 : 4);
+    CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "fan BP transpose: ns too large for LDS");
+    auto kern = bp_transpose_kernel<FanGeo<EXACT>, TW, TH, NB, DET>;
+    if (lds > 64 * 1024)
+        CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(lds)));
+    hipLaunchKernelGGL(kern, dim3(g.na, div_up(nbatch, NB)), dim3(256), lds, st, img, proj, T, L);
+    return check_launch("bp_transpose_kernel<fan>");
 }
 
 int fan_bpt_f32(const ctr_geom& g, const float* img, float* proj, int nbatch, const double* xpos,
                 const double* ypos, const double* sc, double scale, bool det, hipStream_t st)
 {
-    constexpr int TW = 16, TH = 16;
-    FanTLaunch L{};
+    FanProblem P{};
     int lmax = 0;
     bool exact = false;
-    int rc = fan_problem(g, TW, TH, L.P, lmax, exact);
+    int rc = fan_problem(g, 16, 16, P, lmax, exact);
     if (rc) return rc;
-    L.nbatch = nbatch;
-    L.tiles_x = div_up(g.nx, TW);
-    L.tiles_y = div_up(g.ny, TH);
-    L.scale = scale;
-    const size_t rlen = g.ns + 1;
+    const FanTables T{xpos, ypos, sc};
     if (det) {
         // Fixed-point scale from max|img| * max weight * contributions per channel.
         float* partial = nullptr;
@@ -445,43 +301,21 @@ int fan_bpt_f32(const ctr_geom& g, const float* img, float* proj, int nbatch, co
         const size_t n = static_cast<size_t>(nbatch) * g.nx * g.ny;
         const int nblk = static_cast<int>(std::min<size_t>(kMaxBlocks, (n + 255) / 256));
         hipLaunchKernelGGL(abs_max_partial_kernel, dim3(nblk), dim3(256), 0, st, img, n, partial);
-        const double rimg = std::sqrt(std::pow(std::fabs(cent_x(g)) * g.dx + g.nx * g.dx, 2) +
-                                      std::pow(std::fabs(cent_y(g)) * g.dy + g.ny * g.dy, 2));
-        const double wmax = g.dsd / std::max(g.dso - rimg, 1e-3 * g.dso);
+        const double wmax = g.dsd / fan_min_distance(g, 16, 16);
         const double bound = 2.0 * g.nx * g.ny * std::max(1.0, wmax * wmax);
         hipLaunchKernelGGL(fix_scale_kernel, dim3(1), dim3(64), 0, st, partial, nblk, bound, fs);
-        L.fix_scale = fs;
-        constexpr int NB = 4;
-        const size_t lds = kAnchorChunk * sizeof(FanAnchor) + NB * rlen * 8;
-        CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "fan BP transpose: ns too large");
-        dim3 grid(g.na, div_up(nbatch, NB));
-        auto kern = exact ? fan_bpt_kernel<TW, TH, NB, true, true>
-                          : fan_bpt_kernel<TW, TH, NB, false, true>;
-        if (lds > 64 * 1024)
-            CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        static_cast<int>(lds)));
-        hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, img, proj, xpos, ypos, sc, L);
-        rc = check_launch("fan_bpt_kernel<det>");
+        rc = check_launch("fix_scale_kernel");
+        if (rc == CTR_OK)
+            rc = exact ? launch_fan_bpt<true, 4, true>(g, P, nbatch, scale, fs, img, proj, T, st)
+                       : launch_fan_bpt<false, 4, true>(g, P, nbatch, scale, fs, img, proj, T, st);
         CTR_HIP(hipFreeAsync(partial, st));
         CTR_HIP(hipFreeAsync(fs, st));
         return rc;
     }
     const int nb = nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
-    const size_t lds = kAnchorChunk * sizeof(FanAnchor) + static_cast<size_t>(nb) * rlen * 4;
-    CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "fan BP transpose: ns too large");
-    dim3 grid(g.na, div_up(nbatch, nb));
-#define CTR_FANT(NB_)                                                                       \
-    {                                                                                       \
-        auto kern = exact ? fan_bpt_kernel<TW, TH, NB_, true, false>                        \
-                          : fan_bpt_kernel<TW, TH, NB_, false, false>;                      \
-        if (lds > 64 * 1024)                                                                \
-            CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                \
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,         \
-                                        static_cast<int>(lds)));                            \
-        hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, img, proj, xpos, ypos, sc, L);   \
-        return check_launch("fan_bpt_kernel");                                              \
-    }
+#define CTR_FANT(NB_)                                                                           \
+    return exact ? launch_fan_bpt<true, NB_, false>(g, P, nbatch, scale, nullptr, img, proj, T, st) \
+                 : launch_fan_bpt<false, NB_, false>(g, P, nbatch, scale, nullptr, img, proj, T, st);
     switch (nb) {
         case 1: CTR_FANT(1)
         case 2: CTR_FANT(2)

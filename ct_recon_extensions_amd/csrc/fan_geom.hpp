@@ -1,5 +1,6 @@
-// Per-(pixel tile, view) fan-beam geometry shared by the BP kernel and its
-// transpose, so both evaluate bit-identical channel positions and weights.
+// Per-(pixel tile, view) fan-beam geometry, the policy the BP engine uses
+// for bp_fan_2d and bp_fan_2d_angle; the forward kernel and its transpose
+// evaluate bit-identical channel positions and weights through it.
 //
 // For a pixel at (x, y) and source angle beta (source at
 // (-dso sin b, dso cos b)) the reference computes, in FP64
@@ -20,6 +21,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "tile_common.hpp"
+
 namespace ctr {
 
 struct alignas(16) FanAnchor {
@@ -36,49 +39,60 @@ struct FanProblem {
     double dso, dsd, ds, cents, dx, dy;
     int ns, na, nx, ny;
     // polynomial coefficients in v = k*u (k = dsd/ds):
-    //   atan:  ds(v) = v * (1 + v2*(p1 + v2*p2)),   p1 = -1/(3k^2), p2 = 1/(5k^4)
-    //   1/(1+u^2) = 1 + v2*(g1 + v2*g2),             g1 = -1/k^2,   g2 = 1/k^4
+    //   atan:      ds(v) = v * (1 + v2*(p1 + v2*p2)),  p1 = -1/(3k^2), p2 = 1/(5k^4)
+    //   1/(1+u^2) = 1 + v2*(g1 + v2*g2),                 g1 = -1/k^2,   g2 = 1/k^4
     float p1, p2, g1, g2;
-    float k;  // dsd/ds in FP32 (exact-mode only)
+    float k;  // dsd/ds in FP32 (exact mode)
+};
+
+struct FanTables {
+    const double* xpos;     // [nx]   (src/bp_fan_2d.cc:37-40)
+    const double* ypos;     // [ny]
+    const double* sincos;   // [2na]  (sin, cos) of the source angle
 };
 
 // Anchor of the tile whose local pixel offsets span [ilo, ihi] x [jlo, jhi]
-// around (xc, yc) = (xpos[ic], ypos[jc]).  FP64, then rounded to FP32.
+// around (xc, yc).  FP64 for the anchor itself, FP32 for the span bounds.
 __device__ inline FanAnchor fan_anchor(const FanProblem& P, double xc, double yc, double sb,
                                        double cb, int ilo, int ihi, int jlo, int jhi)
 {
     const double d = P.dso + xc * sb - yc * cb;
     const double r = xc * cb + yc * sb;
     const double L2 = d * d + r * r;
+    const double inv = 1.0 / L2;
     const double k = P.dsd / P.ds;
     const double sc = atan2(r, d) * k + P.cents;
     // dd = i*dx*sb + j*dy*cb,  dr = i*dx*cb - j*dy*sb  (y decreases with iy)
-    const double nI = P.dx * (cb * d - sb * r) / L2;
-    const double nJ = -P.dy * (sb * d + cb * r) / L2;
-    const double eI = P.dx * (sb * d + cb * r) / L2;
-    const double eJ = P.dy * (cb * d - sb * r) / L2;
-    // The channel offset is monotone along any line, so the tile's extreme
-    // channels are reached at its corners.
-    double lo = 0.0, hi = 0.0;
+    const double nI = P.dx * (cb * d - sb * r) * inv;
+    const double nJ = -P.dy * (sb * d + cb * r) * inv;
+    const double eI = P.dx * (sb * d + cb * r) * inv;
+    const double eJ = P.dy * (cb * d - sb * r) * inv;
+    // The channel offset is monotone along any line (a ratio of linear
+    // functions through atan), so the tile's extreme channels are at corners.
+    const float fnI = static_cast<float>(nI), fnJ = static_cast<float>(nJ);
+    const float feI = static_cast<float>(eI), feJ = static_cast<float>(eJ);
+    const float fk = static_cast<float>(k);
+    float lo = 0.0f, hi = 0.0f;
     const int ci[2] = {ilo, ihi}, cj[2] = {jlo, jhi};
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const double u = (nI * ci[a] + nJ * cj[b]) / (1.0 + eI * ci[a] + eJ * cj[b]);
-            const double dsv = k * atan(u);
-            lo = fmin(lo, dsv);
-            hi = fmax(hi, dsv);
+            const float u = (fnI * ci[a] + fnJ * cj[b]) / (1.0f + feI * ci[a] + feJ * cj[b]);
+            const float dsv = fk * atanf(u);
+            lo = fminf(lo, dsv);
+            hi = fmaxf(hi, dsv);
         }
     const double smin = sc + lo, smax = sc + hi;
     const int is0 = static_cast<int>(floor(smin)) - 1;
-    const double q = P.dsd / sqrt(L2);  // sqrt(dsd^2 / Lc^2)
+    const double q = P.dsd * sqrt(inv);  // sqrt(dsd^2 / Lc^2)
+    const double iq = 1.0 / q;
     FanAnchor A;
-    A.ai = static_cast<float>(k * nI / q);
-    A.aj = static_cast<float>(k * nJ / q);
-    A.bi = static_cast<float>(eI / q);
-    A.bj = static_cast<float>(eJ / q);
-    A.b0 = static_cast<float>(1.0 / q);
+    A.ai = static_cast<float>(k * nI * iq);
+    A.aj = static_cast<float>(k * nJ * iq);
+    A.bi = static_cast<float>(eI * iq);
+    A.bj = static_cast<float>(eJ * iq);
+    A.b0 = static_cast<float>(iq);
     A.fc = static_cast<float>(sc - is0);
     A.is0 = is0;
     A.inside = (smin >= 0.25 && smax <= P.ns - 1.25) ? 1 : 0;
@@ -107,5 +121,24 @@ __device__ __forceinline__ void fan_pixel(const FanProblem& P, const FanAnchor& 
         w = rd * rd * fmaf(v2, fmaf(v2, P.g2, P.g1), 1.0f);
     }
 }
+
+// Policy for the BP engine.
+template <bool EXACT>
+struct FanGeo {
+    using Problem = FanProblem;
+    using Tables = FanTables;
+    using Anchor = FanAnchor;
+    __device__ static Anchor anchor(const Problem& P, const Tables& T, const TileFrame& F,
+                                    int view, int ilo, int ihi, int jlo, int jhi)
+    {
+        return fan_anchor(P, T.xpos[F.ic], T.ypos[F.jc], T.sincos[2 * view],
+                          T.sincos[2 * view + 1], ilo, ihi, jlo, jhi);
+    }
+    __device__ __forceinline__ static void pixel(const Problem& P, const Anchor& A, float fi,
+                                                 float fj, float& f, float& w)
+    {
+        fan_pixel<EXACT>(P, A, fi, fj, f, w);
+    }
+};
 
 }  // namespace ctr
