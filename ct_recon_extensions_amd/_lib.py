@@ -58,13 +58,13 @@ _SIGS = {
     "ctr_fp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_uint, _P]),
     "ctr_ramp": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, _P]),
     "ctr_fan_weighting": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P]),
+    "ctr_ramp_toeplitz": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, C.c_int, _P]),
 }
 
 # Entry points that only exist once the corresponding sources are built.
 _OPTIONAL = {
     "ctr_fan_bp_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "ctr_ragged_fan_bp": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_int, _P]),
-    "ctr_ramp_toeplitz_bf16": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, C.c_int, _P]),
 }
 
 _lock = threading.Lock()

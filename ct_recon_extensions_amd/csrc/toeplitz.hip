@@ -1,0 +1,249 @@
+// Ramp filter as a Toeplitz GEMM on the gfx950 matrix cores.
+//
+// The filter of src/filter.cc:67-91,  out[r][n] = ds * sum_k x[r][k] h[n-k],
+// has h[m] = 0 for every even m != 0 (src/filter.cc:26-30, :45-49).  Split
+// the channels by parity: with ho[q] = h[2q+1] and pi the parity of n,
+//     out[r][2j+pi] = ds * ( h0 x[r][2j+pi]  +  sum_i x[r][2i+1-pi] ho[j-i+pi-1] ),
+// i.e. for each parity one GEMM  C = A * G  with A the opposite-parity
+// columns of the rows (M = rows, K ~ ns/2) and G[i][j] = ho[j-i+pi-1] a
+// Toeplitz matrix (N ~ ns/2): half the MACs of the dense Toeplitz product,
+// and G is never stored: a lane's B operand is one entry of the tap vector
+// kept in LDS.  Fan / flat weighting (src/fan_weighting.cc:19-35) is fused
+// into the staging of A.
+//
+// FP32 mode (ramp_mfma_kernel<false>): v_mfma_f32_16x16x4_f32, FP32 inputs,
+// FP32 accumulation.
+// BF16x3 mode (ramp_mfma_kernel<true>): v_mfma_f32_16x16x32_bf16 with the
+// operands split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi)), and the
+// compensated product hi*hi + hi*lo + lo*hi accumulated in FP32 (the lo*lo
+// term, ~2^-16 relative, is dropped).
+//
+// Workgroup: 4 waves, a 64-row x 64-output tile of one parity; each wave a
+// 32 x 32 block (2 x 2 MFMA tiles of 16 x 16).  A is staged in LDS in
+// K-chunks of 64 (64 x 64 words, padded); the tap vector covers the whole
+// row.
+#include "ctr_common.hpp"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace ctr {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kTM = 64, kTN = 64, kKC = 64, kAPad = 4;
+
+struct ToeplitzParams {
+    const float* x;      // [nrow][ns]
+    float* out;          // [nrow][ns]
+    const float* taps;   // FP32 h[m] at taps[ns + m], m in [-ns, ns]
+    const float* wts;    // FP32 per-channel weight, or nullptr
+    int ns, nrow;
+    int n_even, n_odd;   // outputs of each parity
+    int tiles_even;      // column tiles of parity 0 (the rest are parity 1)
+    float ds;
+};
+
+__device__ __forceinline__ float wt(const ToeplitzParams& P, int k)
+{
+    return P.wts ? P.wts[k] : 1.0f;
+}
+
+template <bool BF16X3>
+__global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
+{
+    // taps ho[q] at tap[q - qmin] for every q a tile can touch (zero where
+    // |2q+1| > ns): q = j - i + pi - 1 with j < nout + kTN, i < kdim + kKC
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* a_lds = reinterpret_cast<float*>(smem);  // [kTM][kKC + kAPad]
+    float* tap = a_lds + kTM * (kKC + kAPad);
+    const int qmin = -(P.ns / 2) - kKC - 4;
+    const int nq = P.ns + kKC + kTN + 10;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = blockIdx.x * kTM;
+    const int pi = blockIdx.y < P.tiles_even ? 0 : 1;
+    const int j0 = (pi == 0 ? blockIdx.y : blockIdx.y - P.tiles_even) * kTN;
+    const int nout = pi == 0 ? P.n_even : P.n_odd;
+    const int kdim = pi == 0 ? P.ns / 2 : (P.ns + 1) / 2;  // inputs of parity 1-pi
+    const int kpar = 1 - pi;
+
+    for (int e = tid; e < nq; e += 256) {
+        const int q = e + qmin;
+        const int m = 2 * q + 1;
+        tap[e] = (m >= -P.ns && m <= P.ns) ? P.taps[P.ns + m] : 0.0f;
+    }
+
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // wave block in the tile
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int i0 = 0; i0 < kdim; i0 += kKC) {
+        __syncthreads();  // taps written / previous chunk consumed
+        // Stage A: rows r0.., inputs i0.. (x[r][2i + kpar] * w), 16 per thread.
+        float v[kTM * kKC / 256];
+#pragma unroll
+        for (int u = 0; u < kTM * kKC / 256; ++u) {
+            const int e = tid + u * 256;
+            const int rr = e / kKC, ii = e % kKC;
+            const int r = r0 + rr, i = i0 + ii;
+            float val = 0.0f;
+            if (r < P.nrow && i < kdim) {
+                const int k = 2 * i + kpar;
+                val = P.x[static_cast<size_t>(r) * P.ns + k] * wt(P, k);
+            }
+            v[u] = val;
+        }
+#pragma unroll
+        for (int u = 0; u < kTM * kKC / 256; ++u) {
+            const int e = tid + u * 256;
+            a_lds[(e / kKC) * (kKC + kAPad) + e % kKC] = v[u];
+        }
+        __syncthreads();
+        const int tapbase = j0 + wc - i0 + pi - 1 - qmin;  // G[i][j] = ho[j - i + pi - 1]
+        if constexpr (!BF16X3) {
+#pragma unroll 4
+            for (int ik = 0; ik < kKC; ik += 4) {
+                // lane: A[m = lane%16][k = lane/16], B[k = lane/16][n = lane%16]
+                const int m = lane & 15, kk = ik + (lane >> 4);
+                float a[2], b[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) a[t] = a_lds[(wr + 16 * t + m) * (kKC + kAPad) + kk];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) b[t] = tap[tapbase + 16 * t + m - kk];
+#pragma unroll
+                for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+                    for (int tb = 0; tb < 2; ++tb)
+                        acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ta], b[tb],
+                                                                           acc[ta][tb], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int ik = 0; ik < kKC; ik += 32) {
+                // lane: A[m = lane%16][k = 8(lane/16) + e], B[k = 8(lane/16) + e][n = lane%16]
+                const int m = lane & 15, kb = ik + 8 * (lane >> 4);
+                bf16x8 ahi[2], alo[2], bhi[2], blo[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float xa = a_lds[(wr + 16 * t + m) * (kKC + kAPad) + kb + e];
+                        const __bf16 h = static_cast<__bf16>(xa);
+                        ahi[t][e] = h;
+                        alo[t][e] = static_cast<__bf16>(xa - static_cast<float>(h));
+                        const float xb = tap[tapbase + 16 * t + m - (kb + e)];
+                        const __bf16 g = static_cast<__bf16>(xb);
+                        bhi[t][e] = g;
+                        blo[t][e] = static_cast<__bf16>(xb - static_cast<float>(g));
+                    }
+                }
+#pragma unroll
+                for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+                    for (int tb = 0; tb < 2; ++tb) {
+                        acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            alo[ta], bhi[tb], acc[ta][tb], 0, 0, 0);
+                        acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            ahi[ta], blo[tb], acc[ta][tb], 0, 0, 0);
+                        acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            ahi[ta], bhi[tb], acc[ta][tb], 0, 0, 0);
+                    }
+            }
+        }
+    }
+
+    // D layout: lane holds D[4*(lane/16) + t][lane%16], t = 0..3.
+    const float h0 = P.taps[P.ns];
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int r = r0 + wr + 16 * ta + 4 * (lane >> 4) + t;
+                const int j = j0 + wc + 16 * tb + (lane & 15);
+                if (r < P.nrow && j < nout) {
+                    const int n = 2 * j + pi;
+                    const size_t o = static_cast<size_t>(r) * P.ns + n;
+                    const float xw = P.x[o] * wt(P, n);
+                    P.out[o] = P.ds * fmaf(h0, xw, acc[ta][tb][t]);
+                }
+            }
+}
+
+// FP32 taps and (optionally) per-channel cosine weights from the FP64 taps.
+__global__ void toeplitz_prep_kernel(const double* __restrict__ h, float* __restrict__ hf,
+                                     float* __restrict__ w, int ns, int wtype, double ds,
+                                     double dso, double dsd, double cents)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * ns + 1) hf[i] = static_cast<float>(h[i]);
+    if (w && i < ns) {
+        const double s = ds * (static_cast<double>(i) - cents);
+        const double c = wtype == 2 ? cos(atan2(s, dsd)) : cos(s / dsd);
+        w[i] = static_cast<float>(dso * fabs(c) / dsd);
+    }
+}
+
+int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int nrow,
+                  const double* filter, int weight_type, bool bf16x3, hipStream_t st)
+{
+    CTR_REQUIRE(g.ns >= 2, CTR_ERR_ARG, "Toeplitz ramp filter: ns must be >= 2");
+    const int ns = static_cast<int>(g.ns);
+    float* hf = nullptr;
+    CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&hf), (3 * ns + 1) * sizeof(float), st));
+    float* w = weight_type ? hf + 2 * ns + 1 : nullptr;
+    hipLaunchKernelGGL(toeplitz_prep_kernel, dim3(div_up(2 * ns + 1, 256)), dim3(256), 0, st,
+                       filter, hf, w, ns, weight_type, g.ds, g.dso, g.dsd, cent_s(g));
+    int rc = check_launch("toeplitz_prep_kernel");
+    if (rc == CTR_OK) {
+        ToeplitzParams P{};
+        P.x = in;
+        P.out = out;
+        P.taps = hf;
+        P.wts = w;
+        P.ns = ns;
+        P.nrow = static_cast<int>(nrow);
+        P.n_even = (ns + 1) / 2;
+        P.n_odd = ns / 2;
+        P.tiles_even = div_up(P.n_even, kTN);
+        P.ds = static_cast<float>(g.ds);
+        const size_t lds =
+            (static_cast<size_t>(kTM) * (kKC + kAPad) + ns + kKC + kTN + 10) * sizeof(float);
+        CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "Toeplitz ramp filter: ns too large");
+        auto kern = bf16x3 ? ramp_mfma_kernel<true> : ramp_mfma_kernel<false>;
+        if (lds > 64 * 1024)
+            CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        static_cast<int>(lds)));
+        const dim3 grid(div_up(nrow, kTM), P.tiles_even + div_up(P.n_odd, kTN));
+        hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, P);
+        rc = check_launch("ramp_mfma_kernel");
+    }
+    CTR_HIP(hipFreeAsync(hf, st));
+    return rc;
+}
+
+}  // namespace ctr
+
+extern "C" int ctr_ramp_toeplitz(const ctr_geom* g, int dtype, const void* in, void* out,
+                                 unsigned int nrow_total, const double* filter, int weight_type,
+                                 int bf16x3, void* stream)
+{
+    CTR_REQUIRE(g && in && out && filter, CTR_ERR_ARG, "ctr_ramp_toeplitz: null argument");
+    CTR_REQUIRE(dtype == CTR_F32, CTR_ERR_UNSUPPORTED,
+                "ctr_ramp_toeplitz: float32 only (use ctr_ramp for float64)");
+    CTR_REQUIRE(weight_type >= 0 && weight_type <= 2, CTR_ERR_ARG,
+                "ctr_ramp_toeplitz: weight_type must be 0, 1 or 2");
+    if (nrow_total == 0) return CTR_OK;
+    return ctr::ramp_toeplitz(*g, static_cast<const float*>(in), static_cast<float*>(out),
+                              nrow_total, filter, weight_type, bf16x3 != 0,
+                              static_cast<hipStream_t>(stream));
+}

@@ -330,26 +330,55 @@ def parallel_projection_2d_grad(grad_in, img_shape, img_space, proj_shape, chann
 # ------------------------------------------------- ramp filter / weighting
 class _Ramp(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x2, p, taps, wp):
-        ctx.p, ctx.taps, ctx.wp = p, taps, wp
-        return _ramp_launch(x2, p, taps, wp)
+    def forward(ctx, x2, p, taps, impl):
+        ctx.p, ctx.taps, ctx.impl = p, taps, impl
+        return _ramp_launch(x2, p, taps, None, impl)
 
     @staticmethod
     def backward(ctx, g):
-        if ctx.wp is not None:
-            raise RuntimeError("the fused weighting + filter has no backward; compose "
-                               "fan_weighting and ramp_filter instead")
-        return _Ramp.apply(g.contiguous(), ctx.p, ctx.taps, None), None, None, None
+        # the taps are symmetric: the transpose is the same filter
+        return _Ramp.apply(g.contiguous(), ctx.p, ctx.taps, ctx.impl), None, None, None
 
 
-def _ramp_launch(x2, p, taps, wp):
+class _FanFilter(torch.autograd.Function):
+    """Fan weighting then ramp filter in one kernel; backward: filter then
+    weighting (the transpose of the composition)."""
+
+    @staticmethod
+    def forward(ctx, x2, p, taps, wp, impl):
+        ctx.p, ctx.taps, ctx.wp, ctx.impl = p, taps, wp, impl
+        return _ramp_launch(x2, p, taps, wp, impl)
+
+    @staticmethod
+    def backward(ctx, g):
+        y = _ramp_launch(g.contiguous(), ctx.p, ctx.taps, None, ctx.impl)
+        return _FanW.apply(y, ctx.wp), None, None, None, None
+
+
+# Ramp-filter implementation for float32 tensors:
+#   "mfma"   Toeplitz GEMM on the matrix cores, FP32 MFMA (default)
+#   "bf16x3" Toeplitz GEMM on BF16 MFMA with hi/lo split operands
+#   "direct" LDS-staged direct convolution on the vector ALUs
+# float64 tensors always use the direct FP64 kernel.
+RAMP_IMPLS = ("mfma", "bf16x3", "direct")
+default_ramp_impl = "mfma"
+
+
+def _ramp_launch(x2, p, taps, wp, impl=None):
+    impl = impl or default_ramp_impl
+    if impl not in RAMP_IMPLS:
+        raise ValueError(f"ramp filter impl must be one of {RAMP_IMPLS}")
     out = torch.empty_like(x2)
     g = p.ctr()
     wtype = 0
     if wp is not None:
         g.dso, g.dsd, g.offset_s, wtype = wp.dso, wp.dsd, wp.offset, wp.type
-    _lib.call("ctr_ramp", C.byref(g), _dtype_code(x2), _ptr(x2), _ptr(out), x2.shape[0],
-              _ptr(taps[0]), wtype, _stream(x2))
+    if x2.dtype == torch.float32 and impl != "direct":
+        _lib.call("ctr_ramp_toeplitz", C.byref(g), _lib.CTR_F32, _ptr(x2), _ptr(out),
+                  x2.shape[0], _ptr(taps[0]), wtype, int(impl == "bf16x3"), _stream(x2))
+    else:
+        _lib.call("ctr_ramp", C.byref(g), _dtype_code(x2), _ptr(x2), _ptr(out), x2.shape[0],
+                  _ptr(taps[0]), wtype, _stream(x2))
     return out
 
 
@@ -363,7 +392,8 @@ def _rows(x: torch.Tensor, ns: int, name: str):
     return x.reshape(-1, ns).contiguous()
 
 
-def ramp_filter(proj, n_channel, n_row, channel_space, dsd=-1, type="par", window="None"):
+def ramp_filter(proj, n_channel, n_row, channel_space, dsd=-1, type="par", window="None",
+                impl=None):
     """Ramp filter along the channels of every row (python/tf_ctrecon.py:112-127;
     src/filter.cc:67-91).  4-D [N, nrow, ns, 1] or 5-D [N, D, H, ns, 1]
     (D*H = nrow, tensorflow/ramp_filter_ops.cu:130-143)."""
@@ -375,14 +405,14 @@ def ramp_filter(proj, n_channel, n_row, channel_space, dsd=-1, type="par", windo
     if x2.shape[0] % p.nrow:
         raise ValueError(f"ramp_filter: {x2.shape[0]} rows is not a multiple of n_row={p.nrow}")
     taps = device_tables("ramp", p, x2.device)
-    return _Ramp.apply(x2, p, taps, None).reshape(proj.shape)
+    return _Ramp.apply(x2, p, taps, impl).reshape(proj.shape)
 
 
 def ramp_filter_grad(grad_in, n_channel, n_row, channel_space, dsd=-1, type="par",
-                     window="None"):
+                     window="None", impl=None):
     """Transpose of ramp_filter (python/tf_ctrecon.py:129-146; src/filter.cc:96-124);
     the taps are symmetric, so it is the same filter."""
-    return ramp_filter(grad_in, n_channel, n_row, channel_space, dsd, type, window)
+    return ramp_filter(grad_in, n_channel, n_row, channel_space, dsd, type, window, impl)
 
 
 class _FanW(torch.autograd.Function):
@@ -422,11 +452,12 @@ def fan_weighting_grad(grad_in, proj_shape, channel_space, channel_offset=0, dso
 
 
 def fan_filter(proj, proj_shape, channel_space, channel_offset=0, dso=1000, dsd=1500,
-               tp="fan"):
-    """fan_weighting followed by the matching ramp filter, fused into one
-    kernel (the weighting is applied while the filter stages its rows)."""
+               tp="fan", impl=None):
+    """fan_weighting followed by the matching ramp filter (numpy_ext/np_ctrecon.py:91-93),
+    fused into one kernel: the weighting is applied while the filter stages
+    its rows.  Differentiable (backward: filter, then weighting)."""
     wp = _fanw_param(proj_shape, channel_space, channel_offset, dso, dsd, tp)
     p = FilterParam(wp.ns, wp.nrow, wp.ds, float(dsd), wp.type, 0)
     x2 = _rows(proj, p.ns, "proj")
     taps = device_tables("ramp", p, x2.device)
-    return _ramp_launch(x2, p, taps, wp).reshape(proj.shape)
+    return _FanFilter.apply(x2, p, taps, wp, impl).reshape(proj.shape)

@@ -134,6 +134,15 @@ int ctr_fp_t(const ctr_geom* g, int dtype, const void* proj, void* img, int nbat
  * g->offset_s; 0 = none), FanWeighting<T> (src/fan_weighting.cc:19-35). */
 int ctr_ramp(const ctr_geom* g, int dtype, const void* in, void* out, unsigned int nrow_total,
              const double* filter, int weight_type, void* stream);
+/* The same ramp filter (float32) as a Toeplitz GEMM on the matrix cores:
+ * the even taps other than h0 vanish, so each output parity is a product of
+ * the opposite-parity input columns with a Toeplitz matrix of the odd taps.
+ * bf16x3 = 0: FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulation;
+ * bf16x3 = 1: BF16 MFMA on hi/lo split operands, hi*hi + hi*lo + lo*hi
+ * accumulated in FP32.  Same arguments and fused weighting as ctr_ramp. */
+int ctr_ramp_toeplitz(const ctr_geom* g, int dtype, const void* in, void* out,
+                      unsigned int nrow_total, const double* filter, int weight_type, int bf16x3,
+                      void* stream);
 /* Fan / flat cosine weighting and its transpose (the same diagonal),
  * FanWeighting<T> / FanWeightingGrad<T> (src/fan_weighting.cc:19-59). */
 int ctr_fan_weighting(const ctr_geom* g, int dtype, const void* in, void* out,

@@ -316,3 +316,58 @@ def test_autograd_fan_fbp_backward_is_transpose(F, det):
         np.testing.assert_array_equal(host(proj.grad), host(ref))
     else:
         assert rel_l2(host(proj.grad), host(ref)) < 1e-6
+
+
+# ------------------------------------------- ramp filter on the matrix cores
+@pytest.mark.parametrize("ns", [33, 64, 367, 736, 1472])
+@pytest.mark.parametrize("typ,name", [(0, "par"), (1, "fan")])
+def test_ramp_mfma_vs_oracle(F, ns, typ, name):
+    nrow = 37
+    g = O.geom(ns=ns, nrow=nrow, ds=0.65, dsd=1085.6, type=typ)
+    x = randn(ns + typ, (nrow, ns))
+    ref = O.ramp(g, x, O.ramp_prep(g))
+    out = host(F.ramp_filter(cuda(x), ns, nrow, 0.65, 1085.6, name, impl="mfma"))
+    assert rel_l2(out, ref) < TOL32
+
+
+def test_ramp_bf16x3_precision(F):
+    """BF16 hi/lo split (3 products): relative L2 vs the FP64 filter, stated
+    bound 1e-4 (its per-product error is ~2^-16)."""
+    ns, nrow = 736, 64
+    g = O.geom(ns=ns, nrow=nrow, ds=1.3, dsd=1085.6, type=1)
+    x = randn(77, (nrow, ns)).astype(np.float64)
+    ref = O.ramp(g, x, O.ramp_prep(g))
+    out = host(F.ramp_filter(cuda(x, torch.float32), ns, nrow, 1.3, 1085.6, "fan", impl="bf16x3"))
+    err = rel_l2(out, ref)
+    print(f"bf16x3 ramp rel L2 vs FP64: {err:.3e}")
+    assert err < 1e-4
+
+
+@pytest.mark.parametrize("impl", ["mfma", "direct", "bf16x3"])
+def test_fused_fan_filter_impls(F, impl):
+    ns, na = 736, 40
+    x = randn(13, (2, na, ns))
+    kw = dict(proj_shape=[na, ns], channel_space=1.3, channel_offset=0.25, dso=595.0,
+              dsd=1085.6, tp="fan")
+    fused = host(F.fan_filter(cuda(x), **kw, impl=impl))
+    g = O.geom(ns=ns, nrow=na, ds=1.3, offset_s=0.25, dso=595.0, dsd=1085.6, type=1)
+    h = O.ramp_prep(g)
+    tol = 1e-4 if impl == "bf16x3" else TOL32
+    for b in range(2):
+        assert rel_l2(fused[b], O.ramp(g, O.fanw(g, x[b]), h)) < tol
+
+
+def test_fan_filter_backward_is_transpose(F):
+    ns, na = 128, 12
+    kw = dict(proj_shape=[na, ns], channel_space=1.3, channel_offset=0.1, dso=300.0,
+              dsd=500.0, tp="fan")
+    x = cuda(randn(14, (2, na, ns)))
+    y = cuda(randn(15, (2, na, ns)))
+    gap = adjoint_gap(lambda v: F.fan_filter(v, **kw),
+                      lambda v: F.fan_weighting_grad(F.ramp_filter_grad(
+                          v, ns, na, 1.3, 500.0, "fan"), **kw), x, y)
+    assert gap < ADJ
+    xr = x.clone().requires_grad_(True)
+    F.fan_filter(xr, **kw).backward(y)
+    ref = F.fan_weighting_grad(F.ramp_filter_grad(y, ns, na, 1.3, 500.0, "fan"), **kw)
+    assert rel_l2(host(xr.grad), host(ref)) < TOL32
