@@ -70,6 +70,16 @@ def build(verbose: bool = True) -> None:
     if "npext.cpp" in objs:
         _link([objs["npext.cpp"]], os.path.join(LIB, "libnumpy_ctext.so"),
               [f"-L{LIB}", "-lctrecon_hip", "-Wl,-rpath,$ORIGIN"])
+    # C++ driver of the class API for tests/test_class_api.py
+    drv_src = os.path.join(ROOT, "tests", "cpp", "class_api_main.cpp")
+    if os.path.exists(drv_src):
+        drv = os.path.join(ROOT, "tests", "cpp", "class_api_main")
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", f"-I{ROOT}", drv_src,
+               "-o", drv, f"-L{LIB}", "-lctrecon_hip", f"-Wl,-rpath,{LIB}",
+               "-Wl,-rpath,$ORIGIN/../../ct_recon_extensions_amd/lib"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"class API driver build failed:\n{res.stderr}")
     if verbose:
         print(f"built {core} for {ARCH}")
 

@@ -1,0 +1,560 @@
+// The ct_recon:: C++ class API (include/bp_fan_2d.h and siblings) over the
+// C ABI.  Prep classes build the reference's FP64 tables on the host (the
+// reference's accumulation order) and, for calculate_on_gpu, copy them into
+// the caller's device buffers.  Operator classes run one slice on the given
+// stream and synchronise it before returning, like the reference's
+// calculate_on_gpu (e.g. cuda/bp_fan_2d.cu:229).  The operators'
+// calculate_on_cpu is the reference's CPU code, which this GPU library does
+// not provide: it returns false and says so through ctr_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "ctr_common.hpp"
+#include "include/bp_fan_2d.h"
+#include "include/bp_fan_2d_angle.h"
+#include "include/bp_par_2d.h"
+#include "include/fan_weighting.h"
+#include "include/filter.h"
+#include "include/fp_par_2d.h"
+
+namespace ct_recon {
+
+namespace {
+
+template <typename T>
+constexpr int dtype_of()
+{
+    return std::is_same<T, double>::value ? CTR_F64 : CTR_F32;
+}
+
+bool no_cpu(const char* who)
+{
+    ctr::set_error(std::string(who) +
+                   ": the MI355X library has no CPU implementation; use calculate_on_gpu");
+    return false;
+}
+
+bool ok_sync(int rc, hipStream_t st)
+{
+    if (rc != CTR_OK) return false;
+    const hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        ctr::set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
+template <typename V>
+bool to_device(void* dst, const std::vector<V>& src, hipStream_t st)
+{
+    const hipError_t e =
+        hipMemcpyAsync(dst, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+        ctr::set_error(std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
+ctr_geom geom(const FanBackprojection2DParam& p)
+{
+    ctr_geom g{};
+    g.ns = p.ns; g.na = p.na; g.ds = p.ds; g.orbit = p.orbit; g.offset_s = p.offset_s;
+    g.orbit_start = p.orbit_start; g.nx = p.nx; g.ny = p.ny; g.dx = p.dx; g.dy = p.dy;
+    g.offset_x = p.offset_x; g.offset_y = p.offset_y; g.dso = p.dso; g.dsd = p.dsd;
+    g.fov = p.fov;
+    return g;
+}
+
+ctr_geom geom(const FanBackprojection2DAngleParam& p)
+{
+    ctr_geom g{};
+    g.ns = p.ns; g.na = p.na; g.ds = p.ds; g.offset_s = p.offset_s; g.nx = p.nx; g.ny = p.ny;
+    g.dx = p.dx; g.dy = p.dy; g.offset_x = p.offset_x; g.offset_y = p.offset_y; g.dso = p.dso;
+    g.dsd = p.dsd; g.fov = p.fov;
+    return g;
+}
+
+template <class P>
+ctr_geom geom_par(const P& p)
+{
+    ctr_geom g{};
+    g.ns = p.ns; g.na = p.na; g.ds = p.ds; g.orbit = p.orbit; g.offset_s = p.offset_s;
+    g.orbit_start = p.orbit_start; g.nx = p.nx; g.ny = p.ny; g.dx = p.dx; g.dy = p.dy;
+    g.offset_x = p.offset_x; g.offset_y = p.offset_y; g.fov = p.fov;
+    return g;
+}
+
+ctr_geom geom(const FilterParam& p)
+{
+    ctr_geom g{};
+    g.ns = p.ns; g.nrow = p.nrow; g.ds = p.ds; g.dsd = p.dsd; g.type = p.type;
+    return g;
+}
+
+ctr_geom geom(const FanWeightingParam& p)
+{
+    ctr_geom g{};
+    g.ns = p.ns; g.nrow = p.nrow; g.ds = p.ds; g.offset_s = p.offset; g.dso = p.dso;
+    g.dsd = p.dsd; g.type = p.type;
+    return g;
+}
+
+bool fan_tables_gpu(const ctr_geom& g, double* xpos, double* ypos, double* sc, hipStream_t st)
+{
+    std::vector<double> x(g.nx), y(g.ny), s(2 * g.na);
+    if (ctr_fan_tables(&g, x.data(), y.data(), s.data()) != CTR_OK) return false;
+    return to_device(xpos, x, st) && to_device(ypos, y, st) && to_device(sc, s, st) &&
+           ok_sync(CTR_OK, st);
+}
+
+bool fan_angle_tables_gpu(const ctr_geom& g, const double* angles_dev, double* xpos,
+                          double* ypos, double* sc, hipStream_t st)
+{
+    std::vector<double> a(g.na), x(g.nx), y(g.ny), s(2 * g.na);
+    hipError_t e = hipMemcpyAsync(a.data(), angles_dev, g.na * sizeof(double),
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        ctr::set_error(std::string("reading angles: ") + hipGetErrorString(e));
+        return false;
+    }
+    if (ctr_fan_angle_tables(&g, a.data(), x.data(), y.data(), s.data()) != CTR_OK) return false;
+    return to_device(xpos, x, st) && to_device(ypos, y, st) && to_device(sc, s, st) &&
+           ok_sync(CTR_OK, st);
+}
+
+// Temporary device copy of host vectors, released at scope exit.
+struct DevTemp {
+    void* p = nullptr;
+    hipStream_t st;
+    explicit DevTemp(hipStream_t s) : st(s) {}
+    ~DevTemp()
+    {
+        if (p) (void)hipFreeAsync(p, st);
+    }
+    template <typename V>
+    bool put(const std::vector<V>& v)
+    {
+        if (hipMallocAsync(&p, v.size() * sizeof(V), st) != hipSuccess) {
+            ctr::set_error("hipMallocAsync failed");
+            return false;
+        }
+        return to_device(p, v, st);
+    }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------- bp_fan_2d
+bool FanBackprojection2DPixDrivenPrep::calculate_on_cpu(double* xpos, double* ypos,
+                                                        double* sc) const
+{
+    const ctr_geom g = geom(param_);
+    return ctr_fan_tables(&g, xpos, ypos, sc) == CTR_OK;
+}
+
+bool FanBackprojection2DPixDrivenPrep::calculate_on_gpu(double* xpos, double* ypos, double* sc,
+                                                        hipStream_t st) const
+{
+    return fan_tables_gpu(geom(param_), xpos, ypos, sc, st);
+}
+
+bool FanBackprojection2DPixDrivenGradPrep::calculate_on_cpu(double* xpos, double* ypos,
+                                                            double* sc) const
+{
+    const ctr_geom g = geom(param_);
+    return ctr_fan_tables(&g, xpos, ypos, sc) == CTR_OK;
+}
+
+bool FanBackprojection2DPixDrivenGradPrep::calculate_on_gpu(double* xpos, double* ypos,
+                                                            double* sc, hipStream_t st) const
+{
+    return fan_tables_gpu(geom(param_), xpos, ypos, sc, st);
+}
+
+template <typename T>
+bool FanBackprojection2DPixDriven<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                       const double*, const double*) const
+{
+    return no_cpu("FanBackprojection2DPixDriven::calculate_on_cpu");
+}
+
+template <typename T>
+bool FanBackprojection2DPixDriven<T>::calculate_on_gpu(const T* proj, T* img, const double* xpos,
+                                                       const double* ypos, const double* sc,
+                                                       hipStream_t st) const
+{
+    const ctr_geom g = geom(this->param_);
+    return ok_sync(ctr_fan_bp(&g, dtype_of<T>(), proj, img, 1, xpos, ypos, sc,
+                              ctr::fan_bp_scale(g), st),
+                   st);
+}
+
+template <typename T>
+bool FanBackprojection2DPixDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                           const double*, const double*) const
+{
+    return no_cpu("FanBackprojection2DPixDrivenGrad::calculate_on_cpu");
+}
+
+template <typename T>
+bool FanBackprojection2DPixDrivenGrad<T>::calculate_on_gpu(const T* img, T* grad,
+                                                           const double* xpos, const double* ypos,
+                                                           const double* sc, hipStream_t st) const
+{
+    const ctr_geom g = geom(this->param_);
+    return ok_sync(ctr_fan_bp_t(&g, dtype_of<T>(), img, grad, 1, xpos, ypos, sc,
+                                ctr::fan_bp_scale(g), 0, st),
+                   st);
+}
+
+template class FanBackprojection2DPixDriven<float>;
+template class FanBackprojection2DPixDriven<double>;
+template class FanBackprojection2DPixDrivenGrad<float>;
+template class FanBackprojection2DPixDrivenGrad<double>;
+
+// ------------------------------------------------------- bp_fan_2d_angle
+bool FanBackprojection2DAnglePixDrivenPrep::calculate_on_cpu(const double* angles, double* xpos,
+                                                             double* ypos, double* sc) const
+{
+    const ctr_geom g = geom(param_);
+    return ctr_fan_angle_tables(&g, angles, xpos, ypos, sc) == CTR_OK;
+}
+
+bool FanBackprojection2DAnglePixDrivenPrep::calculate_on_gpu(const double* angles, double* xpos,
+                                                             double* ypos, double* sc,
+                                                             hipStream_t st) const
+{
+    return fan_angle_tables_gpu(geom(param_), angles, xpos, ypos, sc, st);
+}
+
+bool FanBackprojection2DAnglePixDrivenGradPrep::calculate_on_cpu(const double* angles,
+                                                                 double* xpos, double* ypos,
+                                                                 double* sc) const
+{
+    const ctr_geom g = geom(param_);
+    return ctr_fan_angle_tables(&g, angles, xpos, ypos, sc) == CTR_OK;
+}
+
+bool FanBackprojection2DAnglePixDrivenGradPrep::calculate_on_gpu(const double* angles,
+                                                                 double* xpos, double* ypos,
+                                                                 double* sc,
+                                                                 hipStream_t st) const
+{
+    return fan_angle_tables_gpu(geom(param_), angles, xpos, ypos, sc, st);
+}
+
+template <typename T>
+bool FanBackprojection2DAnglePixDriven<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                            const double*, const double*) const
+{
+    return no_cpu("FanBackprojection2DAnglePixDriven::calculate_on_cpu");
+}
+
+template <typename T>
+bool FanBackprojection2DAnglePixDriven<T>::calculate_on_gpu(const T* proj, T* img,
+                                                            const double* xpos,
+                                                            const double* ypos,
+                                                            const double* sc,
+                                                            hipStream_t st) const
+{
+    const ctr_geom g = geom(this->param_);
+    return ok_sync(ctr_fan_bp(&g, dtype_of<T>(), proj, img, 1, xpos, ypos, sc, 1.0, st), st);
+}
+
+template <typename T>
+bool FanBackprojection2DAnglePixDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                                const double*,
+                                                                const double*) const
+{
+    return no_cpu("FanBackprojection2DAnglePixDrivenGrad::calculate_on_cpu");
+}
+
+template <typename T>
+bool FanBackprojection2DAnglePixDrivenGrad<T>::calculate_on_gpu(const T* img, T* grad,
+                                                                const double* xpos,
+                                                                const double* ypos,
+                                                                const double* sc,
+                                                                hipStream_t st) const
+{
+    const ctr_geom g = geom(this->param_);
+    return ok_sync(ctr_fan_bp_t(&g, dtype_of<T>(), img, grad, 1, xpos, ypos, sc, 1.0, 0, st),
+                   st);
+}
+
+template class FanBackprojection2DAnglePixDriven<float>;
+template class FanBackprojection2DAnglePixDriven<double>;
+template class FanBackprojection2DAnglePixDrivenGrad<float>;
+template class FanBackprojection2DAnglePixDrivenGrad<double>;
+
+// -------------------------------------------------------------- bp_par_2d
+bool ParallelBackprojection2DPixDrivenPrep::calculate_on_cpu(double* xcos, double* ysin,
+                                                             int*) const
+{
+    const ctr_geom g = geom_par(param_);
+    return ctr_par_tables(&g, xcos, ysin) == CTR_OK;
+}
+
+bool ParallelBackprojection2DPixDrivenPrep::calculate_on_gpu(double* xcos, double* ysin, int*,
+                                                             hipStream_t st) const
+{
+    const ctr_geom g = geom_par(param_);
+    std::vector<double> xc(static_cast<size_t>(g.na) * g.nx), ys(static_cast<size_t>(g.na) * g.ny);
+    if (ctr_par_tables(&g, xc.data(), ys.data()) != CTR_OK) return false;
+    return to_device(xcos, xc, st) && to_device(ysin, ys, st) && ok_sync(CTR_OK, st);
+}
+
+// begins[ia*ns + is], offsets[ia], usex[ia] as src/bp_par_2d.cc:95-145 builds them.
+bool ParallelBackprojection2DPixDrivenGradPrep::calculate_on_cpu(double* begins, double* offsets,
+                                                                 bool* usex) const
+{
+    const ctr_geom g = geom_par(param_);
+    const double cs = ctr::cent_s(g), cx = ctr::cent_x(g), cy = ctr::cent_y(g);
+    const double q = M_PI / 4;
+    double angle = g.orbit_start;
+    for (unsigned int a = 0; a < g.na; ++a) {
+        const double sa = std::sin(angle), ca = std::cos(angle);
+        while (angle < 0) angle += 2 * M_PI;
+        while (angle >= 2 * M_PI) angle -= 2 * M_PI;
+        const bool bx = !((angle >= q && angle < 3 * q) || (angle >= 5 * q && angle < 7 * q));
+        usex[a] = bx;
+        double o1, o2, begin;
+        if (bx) {
+            o1 = g.ds / (ca * g.dx);
+            o2 = g.dy * sa / (ca * g.dx);
+            begin = cx - cy * o2 - cs * o1;
+        } else {
+            o1 = -g.ds / (sa * g.dy);
+            o2 = g.dx * ca / (sa * g.dy);
+            begin = cy - cx * o2 - cs * o1;
+        }
+        offsets[a] = o2;
+        for (unsigned int s = 0; s < g.ns; ++s) {
+            begins[static_cast<size_t>(a) * g.ns + s] = begin;
+            begin += o1;
+        }
+        angle += g.orbit;
+    }
+    return true;
+}
+
+bool ParallelBackprojection2DPixDrivenGradPrep::calculate_on_gpu(double* begins, double* offsets,
+                                                                 bool* usex,
+                                                                 hipStream_t st) const
+{
+    const ctr_geom g = geom_par(param_);
+    std::vector<double> b(static_cast<size_t>(g.na) * g.ns), o(g.na);
+    std::vector<unsigned char> u(g.na);
+    if (!calculate_on_cpu(b.data(), o.data(), reinterpret_cast<bool*>(u.data()))) return false;
+    return to_device(begins, b, st) && to_device(offsets, o, st) && to_device(usex, u, st) &&
+           ok_sync(CTR_OK, st);
+}
+
+template <typename T>
+bool ParallelBackprojection2DPixDriven<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                            const double*, const int*) const
+{
+    return no_cpu("ParallelBackprojection2DPixDriven::calculate_on_cpu");
+}
+
+template <typename T>
+bool ParallelBackprojection2DPixDriven<T>::calculate_on_gpu(const T* proj, T* img,
+                                                            const double* xcos,
+                                                            const double* ysin, const int*,
+                                                            hipStream_t st) const
+{
+    const ctr_geom g = geom_par(this->param_);
+    return ok_sync(ctr_par_bp(&g, dtype_of<T>(), proj, img, 1, xcos, ysin, st), st);
+}
+
+template <typename T>
+bool ParallelBackprojection2DPixDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                                const double*,
+                                                                const bool*) const
+{
+    return no_cpu("ParallelBackprojection2DPixDrivenGrad::calculate_on_cpu");
+}
+
+// The exact transpose needs the forward tables, built here per call.
+template <typename T>
+bool ParallelBackprojection2DPixDrivenGrad<T>::calculate_on_gpu(const T* img, T* grad,
+                                                                const double*, const double*,
+                                                                const bool*,
+                                                                hipStream_t st) const
+{
+    const ctr_geom g = geom_par(this->param_);
+    std::vector<double> xc(static_cast<size_t>(g.na) * g.nx), ys(static_cast<size_t>(g.na) * g.ny);
+    if (ctr_par_tables(&g, xc.data(), ys.data()) != CTR_OK) return false;
+    DevTemp dx(st), dy(st);
+    if (!dx.put(xc) || !dy.put(ys)) return false;
+    return ok_sync(ctr_par_bp_t(&g, dtype_of<T>(), img, grad, 1, static_cast<double*>(dx.p),
+                                static_cast<double*>(dy.p), 0, st),
+                   st);
+}
+
+template class ParallelBackprojection2DPixDriven<float>;
+template class ParallelBackprojection2DPixDriven<double>;
+template class ParallelBackprojection2DPixDrivenGrad<float>;
+template class ParallelBackprojection2DPixDrivenGrad<double>;
+
+// -------------------------------------------------------------- fp_par_2d
+bool ParallelProjection2DRayDrivenPrepare::calculate_on_cpu(double* sc, double* bo,
+                                                            int* usex) const
+{
+    const ctr_geom g = geom_par(param_);
+    return ctr_fp_tables(&g, sc, bo, usex) == CTR_OK;
+}
+
+bool ParallelProjection2DRayDrivenPrepare::calculate_on_gpu(double* sc, double* bo, int* usex,
+                                                            hipStream_t st) const
+{
+    const ctr_geom g = geom_par(param_);
+    std::vector<double> s(2 * g.na), b(2 * static_cast<size_t>(g.na) * g.ns);
+    std::vector<int> u(g.na);
+    if (ctr_fp_tables(&g, s.data(), b.data(), u.data()) != CTR_OK) return false;
+    return to_device(sc, s, st) && to_device(bo, b, st) && to_device(usex, u, st) &&
+           ok_sync(CTR_OK, st);
+}
+
+bool ParallelProjection2DRayDrivenGradPrep::calculate_on_cpu(double* w, double* pos,
+                                                             int* usex) const
+{
+    const ctr_geom g = geom_par(param_);
+    return ctr_fp_grad_tables(&g, w, pos, usex, g.nx) == CTR_OK;
+}
+
+bool ParallelProjection2DRayDrivenGradPrep::calculate_on_gpu(double* w, double* pos, int* usex,
+                                                             hipStream_t st) const
+{
+    const ctr_geom g = geom_par(param_);
+    std::vector<double> wv(g.na), pv(2 * static_cast<size_t>(g.na) * g.nx);
+    std::vector<int> u(g.na);
+    if (ctr_fp_grad_tables(&g, wv.data(), pv.data(), u.data(), g.nx) != CTR_OK) return false;
+    return to_device(w, wv, st) && to_device(pos, pv, st) && to_device(usex, u, st) &&
+           ok_sync(CTR_OK, st);
+}
+
+template <typename T>
+bool ParallelProjection2DRayDriven<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                        const double*, const int*) const
+{
+    return no_cpu("ParallelProjection2DRayDriven::calculate_on_cpu");
+}
+
+template <typename T>
+bool ParallelProjection2DRayDriven<T>::calculate_on_gpu(const T* img, T* proj, const double* sc,
+                                                        const double* bo, const int* usex,
+                                                        hipStream_t st) const
+{
+    const ctr_geom g = geom_par(this->param_);
+    return ok_sync(ctr_fp(&g, dtype_of<T>(), img, proj, 1, sc, bo, usex, st), st);
+}
+
+template <typename T>
+bool ParallelProjection2DRayDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                            const double*, const int*) const
+{
+    return no_cpu("ParallelProjection2DRayDrivenGrad::calculate_on_cpu");
+}
+
+template <typename T>
+bool ParallelProjection2DRayDrivenGrad<T>::calculate_on_gpu(const T* proj, T* img,
+                                                            const double* w, const double* pos,
+                                                            const int* usex,
+                                                            hipStream_t st) const
+{
+    const ctr_geom g = geom_par(this->param_);
+    return ok_sync(ctr_fp_t(&g, dtype_of<T>(), proj, img, 1, w, pos, usex, g.nx, st), st);
+}
+
+template class ParallelProjection2DRayDriven<float>;
+template class ParallelProjection2DRayDriven<double>;
+template class ParallelProjection2DRayDrivenGrad<float>;
+template class ParallelProjection2DRayDrivenGrad<double>;
+
+// ----------------------------------------------------------------- filter
+bool RampFilterPrep::calculate_on_cpu(double* filter) const
+{
+    const ctr_geom g = geom(param_);
+    return ctr_ramp_taps(&g, filter) == CTR_OK;
+}
+
+bool RampFilterPrep::calculate_on_gpu(double* filter, hipStream_t st) const
+{
+    const ctr_geom g = geom(param_);
+    std::vector<double> h(2 * static_cast<size_t>(g.ns) + 1);
+    if (ctr_ramp_taps(&g, h.data()) != CTR_OK) return false;
+    return to_device(filter, h, st) && ok_sync(CTR_OK, st);
+}
+
+template <typename T>
+bool RampFilter<T>::calculate_on_cpu(const T*, const double*, T*) const
+{
+    return no_cpu("RampFilter::calculate_on_cpu");
+}
+
+template <typename T>
+bool RampFilter<T>::calculate_on_gpu(const T* in, const double* filter, T* out,
+                                     hipStream_t st) const
+{
+    const ctr_geom g = geom(param_);
+    return ok_sync(ctr_ramp(&g, dtype_of<T>(), in, out, g.nrow, filter, 0, st), st);
+}
+
+template <typename T>
+bool RampFilterGrad<T>::calculate_on_cpu(const T*, const double*, T*) const
+{
+    return no_cpu("RampFilterGrad::calculate_on_cpu");
+}
+
+template <typename T>
+bool RampFilterGrad<T>::calculate_on_gpu(const T* in, const double* filter, T* out,
+                                         hipStream_t st) const
+{
+    const ctr_geom g = geom(param_);
+    return ok_sync(ctr_ramp(&g, dtype_of<T>(), in, out, g.nrow, filter, 0, st), st);
+}
+
+template class RampFilter<float>;
+template class RampFilter<double>;
+template class RampFilterGrad<float>;
+template class RampFilterGrad<double>;
+
+// --------------------------------------------------------- fan_weighting
+template <typename T>
+bool FanWeighting<T>::calculate_on_cpu(const T*, T*) const
+{
+    return no_cpu("FanWeighting::calculate_on_cpu");
+}
+
+template <typename T>
+bool FanWeighting<T>::calculate_on_gpu(const T* in, T* out, hipStream_t st) const
+{
+    const ctr_geom g = geom(param_);
+    return ok_sync(ctr_fan_weighting(&g, dtype_of<T>(), in, out, g.nrow, st), st);
+}
+
+template <typename T>
+bool FanWeightingGrad<T>::calculate_on_cpu(const T*, T*) const
+{
+    return no_cpu("FanWeightingGrad::calculate_on_cpu");
+}
+
+template <typename T>
+bool FanWeightingGrad<T>::calculate_on_gpu(const T* in, T* out, hipStream_t st) const
+{
+    const ctr_geom g = geom(param_);
+    return ok_sync(ctr_fan_weighting(&g, dtype_of<T>(), in, out, g.nrow, st), st);
+}
+
+template class FanWeighting<float>;
+template class FanWeighting<double>;
+template class FanWeightingGrad<float>;
+template class FanWeightingGrad<double>;
+
+}  // namespace ct_recon

@@ -1,0 +1,198 @@
+// Drives the ct_recon:: class API the way a user of the reference's C++
+// library does (Prep -> operator -> Grad, device buffers, a stream), for
+// tests/test_class_api.py.  Usage:
+//   class_api_main cpu <dir>   tables from every Prep::calculate_on_cpu
+//   class_api_main gpu <dir>   every operator on the GPU, float and double
+// Inputs are read from <dir>/*.in.f64 and outputs written to <dir>/*.f64 as
+// raw little-endian doubles; geometries are fixed and mirrored in the test.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "include/bp_fan_2d.h"
+#include "include/bp_fan_2d_angle.h"
+#include "include/bp_par_2d.h"
+#include "include/fan_weighting.h"
+#include "include/filter.h"
+#include "include/fp_par_2d.h"
+
+using namespace ct_recon;
+
+static std::string g_dir;
+
+static std::vector<double> load(const std::string& name, size_t n)
+{
+    std::vector<double> v(n);
+    FILE* f = std::fopen((g_dir + "/" + name + ".in.f64").c_str(), "rb");
+    if (!f || std::fread(v.data(), 8, n, f) != n) {
+        std::fprintf(stderr, "cannot read %s\n", name.c_str());
+        std::exit(2);
+    }
+    std::fclose(f);
+    return v;
+}
+
+template <typename V>
+static void save(const std::string& name, const std::vector<V>& v)
+{
+    std::vector<double> d(v.begin(), v.end());
+    FILE* f = std::fopen((g_dir + "/" + name + ".f64").c_str(), "wb");
+    std::fwrite(d.data(), 8, d.size(), f);
+    std::fclose(f);
+}
+
+#define CHECK(x)                                                            \
+    do {                                                                    \
+        if (!(x)) {                                                         \
+            std::fprintf(stderr, "FAILED: %s (line %d)\n", #x, __LINE__);   \
+            return 1;                                                       \
+        }                                                                   \
+    } while (0)
+
+template <typename T>
+struct Dev {
+    T* p = nullptr;
+    size_t n;
+    explicit Dev(size_t n) : n(n) { (void)hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T)); }
+    Dev(const std::vector<double>& h) : Dev(h.size())
+    {
+        std::vector<T> t(h.begin(), h.end());
+        (void)hipMemcpy(p, t.data(), n * sizeof(T), hipMemcpyHostToDevice);
+    }
+    ~Dev() { (void)hipFree(p); }
+    std::vector<T> get() const
+    {
+        std::vector<T> h(n);
+        (void)hipMemcpy(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost);
+        return h;
+    }
+};
+
+// Geometries (mirrored in tests/test_class_api.py).
+static const FanBackprojection2DParam kFan(41, 48, 1.7, 2 * M_PI / 48, 0.4, 0.3, 24, 20, 1.3, 1.5,
+                                          0.3, -0.2, 60.0, 110.0, -1.0);
+static const FanBackprojection2DAngleParam kFanA(41, 48, 1.7, 0.4, 24, 20, 1.3, 1.5, 0.3, -0.2,
+                                                 60.0, 110.0, -1.0);
+static const ParallelBackprojection2DParam kPar(41, 36, 0.8, M_PI / 36, -0.35, 0.1, 22, 26, 0.7,
+                                                0.9, -0.4, 0.25, -1.0);
+static const ParallelProjection2DParam kFp(41, 40, 1.0, M_PI / 40, 0.3, 0.0, 24, 24, 1.0, 1.0,
+                                           0.2, -0.1, -1.0);
+static const FilterParam kFilt(33, 5, 0.9, 40.0, 1, 0);
+static const FanWeightingParam kFw(37, 4, 1.3, 0.6, 595.0, 1085.6, 1);
+
+static int cpu()
+{
+    std::vector<double> x(kFan.nx), y(kFan.ny), sc(2 * kFan.na);
+    CHECK(FanBackprojection2DPixDrivenPrep(kFan).calculate_on_cpu(x.data(), y.data(), sc.data()));
+    save("fan_xpos", x); save("fan_ypos", y); save("fan_sincos", sc);
+    std::vector<double> xc(kPar.na * kPar.nx), ys(kPar.na * kPar.ny);
+    CHECK(ParallelBackprojection2DPixDrivenPrep(kPar).calculate_on_cpu(xc.data(), ys.data(), nullptr));
+    save("par_xcos", xc); save("par_ysin", ys);
+    std::vector<double> s2(2 * kFp.na), bo(2 * kFp.na * kFp.ns);
+    std::vector<int> u(kFp.na);
+    CHECK(ParallelProjection2DRayDrivenPrepare(kFp).calculate_on_cpu(s2.data(), bo.data(), u.data()));
+    save("fp_sincos", s2); save("fp_beginoffset", bo); save("fp_usex", u);
+    std::vector<double> w(kFp.na), pos(2 * kFp.na * kFp.nx);
+    CHECK(ParallelProjection2DRayDrivenGradPrep(kFp).calculate_on_cpu(w.data(), pos.data(), u.data()));
+    save("fpg_weights", w); save("fpg_pos", pos); save("fpg_usex", u);
+    std::vector<double> h(2 * kFilt.ns + 1);
+    CHECK(RampFilterPrep(kFilt).calculate_on_cpu(h.data()));
+    save("ramp_taps", h);
+    // operators have no CPU path in this library
+    std::vector<float> pf(kFan.na * kFan.ns), imf(kFan.nx * kFan.ny);
+    CHECK(!FanBackprojection2DPixDriven<float>(kFan).calculate_on_cpu(pf.data(), imf.data(), x.data(), y.data(), sc.data()));
+    std::puts("cpu ok");
+    return 0;
+}
+
+template <typename T>
+static int gpu_typed(const char* suf, hipStream_t st)
+{
+    const std::string s = suf;
+    {   // fan BP and its transpose
+        Dev<double> x(kFan.nx), y(kFan.ny), sc(2 * kFan.na);
+        CHECK(FanBackprojection2DPixDrivenPrep(kFan).calculate_on_gpu(x.p, y.p, sc.p, st));
+        Dev<T> proj(load("fan_proj", kFan.na * kFan.ns)), img(kFan.nx * kFan.ny);
+        CHECK(FanBackprojection2DPixDriven<T>(kFan).calculate_on_gpu(proj.p, img.p, x.p, y.p, sc.p, st));
+        save("fan_bp_" + s, img.get());
+        Dev<double> gx(kFan.nx), gy(kFan.ny), gsc(2 * kFan.na);
+        CHECK(FanBackprojection2DPixDrivenGradPrep(kFan).calculate_on_gpu(gx.p, gy.p, gsc.p, st));
+        Dev<T> im2(load("fan_img", kFan.nx * kFan.ny)), grad(kFan.na * kFan.ns);
+        CHECK(FanBackprojection2DPixDrivenGrad<T>(kFan).calculate_on_gpu(im2.p, grad.p, gx.p, gy.p, gsc.p, st));
+        save("fan_bpt_" + s, grad.get());
+    }
+    {   // fan BP at given angles
+        Dev<double> ang(load("fana_angles", kFanA.na));
+        Dev<double> x(kFanA.nx), y(kFanA.ny), sc(2 * kFanA.na);
+        CHECK(FanBackprojection2DAnglePixDrivenPrep(kFanA).calculate_on_gpu(ang.p, x.p, y.p, sc.p, st));
+        Dev<T> proj(load("fan_proj", kFanA.na * kFanA.ns)), img(kFanA.nx * kFanA.ny);
+        CHECK(FanBackprojection2DAnglePixDriven<T>(kFanA).calculate_on_gpu(proj.p, img.p, x.p, y.p, sc.p, st));
+        save("fana_bp_" + s, img.get());
+    }
+    {   // parallel BP and its transpose
+        Dev<double> xc(kPar.na * kPar.nx), ys(kPar.na * kPar.ny);
+        CHECK(ParallelBackprojection2DPixDrivenPrep(kPar).calculate_on_gpu(xc.p, ys.p, nullptr, st));
+        Dev<T> proj(load("par_proj", kPar.na * kPar.ns)), img(kPar.nx * kPar.ny);
+        CHECK(ParallelBackprojection2DPixDriven<T>(kPar).calculate_on_gpu(proj.p, img.p, xc.p, ys.p, nullptr, st));
+        save("par_bp_" + s, img.get());
+        Dev<double> b(kPar.na * kPar.ns), o(kPar.na);
+        Dev<bool> u(kPar.na);
+        CHECK(ParallelBackprojection2DPixDrivenGradPrep(kPar).calculate_on_gpu(b.p, o.p, u.p, st));
+        Dev<T> im2(load("par_img", kPar.nx * kPar.ny)), grad(kPar.na * kPar.ns);
+        CHECK(ParallelBackprojection2DPixDrivenGrad<T>(kPar).calculate_on_gpu(im2.p, grad.p, b.p, o.p, u.p, st));
+        save("par_bpt_" + s, grad.get());
+    }
+    {   // ray-driven FP and its transpose
+        Dev<double> sc(2 * kFp.na), bo(2 * kFp.na * kFp.ns);
+        Dev<int> u(kFp.na);
+        CHECK(ParallelProjection2DRayDrivenPrepare(kFp).calculate_on_gpu(sc.p, bo.p, u.p, st));
+        Dev<T> img(load("fp_img", kFp.nx * kFp.ny)), proj(kFp.na * kFp.ns);
+        CHECK(ParallelProjection2DRayDriven<T>(kFp).calculate_on_gpu(img.p, proj.p, sc.p, bo.p, u.p, st));
+        save("fp_" + s, proj.get());
+        Dev<double> w(kFp.na), pos(2 * kFp.na * kFp.nx);
+        Dev<int> ug(kFp.na);
+        CHECK(ParallelProjection2DRayDrivenGradPrep(kFp).calculate_on_gpu(w.p, pos.p, ug.p, st));
+        Dev<T> pr2(load("fp_proj", kFp.na * kFp.ns)), img2(kFp.nx * kFp.ny);
+        CHECK(ParallelProjection2DRayDrivenGrad<T>(kFp).calculate_on_gpu(pr2.p, img2.p, w.p, pos.p, ug.p, st));
+        save("fpt_" + s, img2.get());
+    }
+    {   // ramp filter, its transpose, fan weighting
+        Dev<double> h(2 * kFilt.ns + 1);
+        CHECK(RampFilterPrep(kFilt).calculate_on_gpu(h.p, st));
+        Dev<T> in(load("ramp_in", kFilt.nrow * kFilt.ns)), out(kFilt.nrow * kFilt.ns);
+        CHECK(RampFilter<T>(kFilt).calculate_on_gpu(in.p, h.p, out.p, st));
+        save("ramp_" + s, out.get());
+        CHECK(RampFilterGrad<T>(kFilt).calculate_on_gpu(in.p, h.p, out.p, st));
+        save("rampg_" + s, out.get());
+        Dev<T> wi(load("fw_in", kFw.nrow * kFw.ns)), wo(kFw.nrow * kFw.ns);
+        CHECK(FanWeighting<T>(kFw).calculate_on_gpu(wi.p, wo.p, st));
+        save("fw_" + s, wo.get());
+        CHECK(FanWeightingGrad<T>(kFw).calculate_on_gpu(wi.p, wo.p, st));
+        save("fwg_" + s, wo.get());
+    }
+    return 0;
+}
+
+static int gpu()
+{
+    hipStream_t st;
+    CHECK(hipStreamCreate(&st) == hipSuccess);
+    if (gpu_typed<float>("f32", st) || gpu_typed<double>("f64", st)) return 1;
+    CHECK(hipStreamDestroy(st) == hipSuccess);
+    std::puts("gpu ok");
+    return 0;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: %s cpu|gpu <dir>\n", argv[0]);
+        return 2;
+    }
+    g_dir = argv[2];
+    return std::strcmp(argv[1], "cpu") == 0 ? cpu() : gpu();
+}

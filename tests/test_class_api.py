@@ -1,0 +1,140 @@
+"""The reference-compatible boundaries: the ct_recon:: C++ class API (through
+the tests/cpp/class_api_main driver) and the numpy C ABI with its
+FanFBP2DAngle front-end, checked against the oracle."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import ROOT, randn, rel_l2
+
+DRIVER = os.path.join(ROOT, "tests", "cpp", "class_api_main")
+
+# Geometries hard-wired in tests/cpp/class_api_main.cpp.
+FAN = dict(ns=41, na=48, ds=1.7, orbit=2 * math.pi / 48, offset_s=0.4, orbit_start=0.3, nx=24,
+           ny=20, dx=1.3, dy=1.5, offset_x=0.3, offset_y=-0.2, dso=60.0, dsd=110.0)
+PAR = dict(ns=41, na=36, ds=0.8, orbit=math.pi / 36, offset_s=-0.35, orbit_start=0.1, nx=22,
+           ny=26, dx=0.7, dy=0.9, offset_x=-0.4, offset_y=0.25)
+FP = dict(ns=41, na=40, ds=1.0, orbit=math.pi / 40, offset_s=0.3, orbit_start=0.0, nx=24, ny=24,
+          dx=1.0, dy=1.0, offset_x=0.2, offset_y=-0.1)
+FILT = dict(ns=33, nrow=5, ds=0.9, dsd=40.0, type=1)
+FW = dict(ns=37, nrow=4, ds=1.3, offset_s=0.6, dso=595.0, dsd=1085.6, type=1)
+
+
+def _read(d, name):
+    return np.fromfile(os.path.join(d, name + ".f64"), dtype=np.float64)
+
+
+def _write(d, name, a):
+    np.ascontiguousarray(a, np.float64).tofile(os.path.join(d, name + ".in.f64"))
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(DRIVER):
+        pytest.fail("tests/cpp/class_api_main not built: python -m ct_recon_extensions_amd.build")
+    return DRIVER
+
+
+def test_class_api_prep_tables_cpu(driver, tmp_path):
+    r = subprocess.run([driver, "cpu", str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    for a, name in zip(O.fan_prep(O.geom(**FAN)), ("fan_xpos", "fan_ypos", "fan_sincos")):
+        np.testing.assert_array_equal(_read(tmp_path, name), a)
+    for a, name in zip(O.par_prep(O.geom(**PAR)), ("par_xcos", "par_ysin")):
+        np.testing.assert_array_equal(_read(tmp_path, name), a)
+    for a, name in zip(O.fp_prep(O.geom(**FP)), ("fp_sincos", "fp_beginoffset", "fp_usex")):
+        np.testing.assert_array_equal(_read(tmp_path, name), a)
+    for a, name in zip(O.fp_grad_prep(O.geom(**FP)), ("fpg_weights", "fpg_pos", "fpg_usex")):
+        np.testing.assert_array_equal(_read(tmp_path, name), a)
+    np.testing.assert_array_equal(_read(tmp_path, "ramp_taps"), O.ramp_prep(O.geom(**FILT)))
+
+
+@pytest.mark.gpu
+def test_class_api_operators_gpu(driver, tmp_path):
+    gf, gp, gq = O.geom(**FAN), O.geom(**PAR), O.geom(**FP)
+    inputs = {
+        "fan_proj": randn(1, (gf.na, gf.ns), np.float64),
+        "fan_img": randn(2, (gf.ny, gf.nx), np.float64),
+        "fana_angles": np.sort(np.random.default_rng(3).uniform(0, 2 * math.pi, gf.na)),
+        "par_proj": randn(4, (gp.na, gp.ns), np.float64),
+        "par_img": randn(5, (gp.ny, gp.nx), np.float64),
+        "fp_img": randn(6, (gq.ny, gq.nx), np.float64),
+        "fp_proj": randn(7, (gq.na, gq.ns), np.float64),
+        "ramp_in": randn(8, (FILT["nrow"], FILT["ns"]), np.float64),
+        "fw_in": randn(9, (FW["nrow"], FW["ns"]), np.float64),
+    }
+    for k, v in inputs.items():
+        _write(tmp_path, k, v)
+    r = subprocess.run([driver, "gpu", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    tf = O.fan_prep(gf)
+    ta = O.fan_angle_prep(gf, inputs["fana_angles"])
+    tp = O.par_prep(gp)
+    gr = O.geom(**FILT)
+    h = O.ramp_prep(gr)
+    gw = O.geom(**FW)
+    for suf, dt, tol in (("f32", np.float32, 1e-5), ("f64", np.float64, 1e-12)):
+        ref = {
+            "fan_bp": O.fan_bp(gf, inputs["fan_proj"].astype(dt), tf),
+            "fan_bpt": O.fan_bpt(gf, inputs["fan_img"].astype(dt), tf),
+            "fana_bp": O.fan_bp(gf, inputs["fan_proj"].astype(dt), ta, scaled=False),
+            "par_bp": O.par_bp(gp, inputs["par_proj"].astype(dt), tp),
+            "par_bpt": O.par_bpt(gp, inputs["par_img"].astype(dt), tp),
+            "fp": O.fp(gq, inputs["fp_img"].astype(dt), O.fp_prep(gq)),
+            "fpt": O.fp_grad(gq, inputs["fp_proj"].astype(dt), O.fp_grad_prep(gq)),
+            "ramp": O.ramp(gr, inputs["ramp_in"].astype(dt), h),
+            "rampg": O.ramp(gr, inputs["ramp_in"].astype(dt), h, grad=True),
+            "fw": O.fanw(gw, inputs["fw_in"].astype(dt)),
+            "fwg": O.fanw(gw, inputs["fw_in"].astype(dt), grad=True),
+        }
+        for name, want in ref.items():
+            got = _read(tmp_path, f"{name}_{suf}")
+            # the float transposes scatter with atomics; the FP64 ones too
+            t = 1e-5 if (suf == "f32" or name in ("fan_bpt", "par_bpt")) else tol
+            assert rel_l2(got, want.ravel()) < t, (name, suf, rel_l2(got, want.ravel()))
+
+
+# ------------------------------------------------------------ numpy C ABI
+def _nplib():
+    from ct_recon_extensions_amd import np_ctrecon
+    path = os.path.join(ROOT, "ct_recon_extensions_amd", "lib", "libnumpy_ctext.so")
+    if not os.path.exists(path):
+        pytest.fail("libnumpy_ctext.so not built")
+    return np_ctrecon._load()
+
+
+def test_numpy_abi_error_paths():
+    lib = _nplib()
+    assert lib.set_device(-1) is False           # no CPU mode
+    assert lib.fan_weighting_create(0, 4, 1.0, 0.0, 10.0, 20.0, 1) == 0
+    assert lib.ramp_filter_create(8, 2, 1.0, 10.0, 7) == 0
+    assert lib.fan_weighting_destroy(12345) is False
+    h = lib.ramp_filter_create(8, 2, 1.0, 10.0, 0)
+    assert h > 0
+    x = np.zeros((2, 8))
+    assert lib.ramp_filter_run(h, x, x) is False  # no device selected
+    assert lib.ramp_filter_destroy(h) is True
+    assert lib.ramp_filter_destroy(h) is False
+
+
+@pytest.mark.gpu
+def test_numpy_fan_fbp_angle_matches_oracle():
+    from ct_recon_extensions_amd.np_ctrecon import FanFBP2DAngle
+    geo = dict(ns=96, na=60, ds=1.3, offset_s=0.25, nx=32, ny=30, dx=1.6, dy=1.5, offset_x=0.2,
+               offset_y=-0.1, dso=120.0, dsd=220.0)
+    rng = np.random.default_rng(17)
+    angles = np.sort(rng.uniform(0, 2 * math.pi, geo["na"]))
+    param = dict(geo, device=0, fov=-1.0, orbit=2 * math.pi / geo["na"], angles=angles)
+    op = FanFBP2DAngle(param)
+    proj = rng.standard_normal((geo["na"], geo["ns"]))
+    keep = proj.copy()
+    recon = op.run(proj)
+    np.testing.assert_array_equal(proj, keep)    # input not modified
+    g = O.geom(**geo, type=1, nrow=geo["na"])
+    ref = O.fan_bp(g, O.ramp(g, O.fanw(g, proj), O.ramp_prep(g)),
+                   O.fan_angle_prep(g, angles), scaled=False) * param["orbit"]
+    assert rel_l2(recon, ref) < 1e-12
