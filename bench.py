@@ -36,11 +36,17 @@ PEAK_HBM_GBS = 8000.0
 CU, LANES, CLK = 256, 64, 2.4e9
 PEAK_VALU_SLOTS = CU * LANES * CLK  # FP32 VALU lane-issues per second (3.93e13)
 
-# Algorithmic VALU issue slots of the fan BP: geometry per
-# pixel*view, shared by every slice of a batch group, and per slice.
-FAN_GEOM_SLOTS = 23
-FAN_SLICE_SLOTS = 3
-SURVEY_FAN_SLOTS = 28  # per-slice model of the survey (no batching)
+# Algorithmic FP32 VALU issue slots of the fan BP, counting a quarter-rate
+# transcendental (v_rcp_f32) as 4 slots.  Geometry, once per pixel*view and
+# shared by every slice the launch carries: numerator 2, denominator 2, rcp 4,
+# v = num*rcp 1, v^2 1, atan series 2, s = s_c + ... 2, weight 4, floor 1,
+# fraction 1, int convert 1, clamp 1, two tap weights 2, tap address 2 = 26.
+# Per slice: two FMAs into the accumulator = 2.
+FAN_GEOM_SLOTS = 26
+FAN_SLICE_SLOTS = 2
+# Per-slice model with no geometry sharing (atan2 + divide per pixel*view,
+# the textbook count for one slice at a time).
+UNBATCHED_FAN_SLOTS = 28
 
 CFG = dict(nbatch=16, n=512, dx=0.7, na=720, ns=736, ds=1.30, dso=595.0, dsd=1085.6,
            i0=1e5, ellipses=10, seed=20261202)
@@ -223,7 +229,7 @@ def main():
                     "frac": round(hbm_achieved / PEAK_HBM_GBS, 5),
                     "algorithmic_bytes": hbm_bytes},
             "bp_pixel_views_per_s": bp_rate,
-            "survey_28slot_model_frac": round(bp_rate * SURVEY_FAN_SLOTS / PEAK_VALU_SLOTS, 4),
+            "unbatched_28slot_model_frac": round(bp_rate * UNBATCHED_FAN_SLOTS / PEAK_VALU_SLOTS, 4),
         },
         "plain_per_slice_gpu": plain,
     }
