@@ -48,6 +48,10 @@ FAN_SLICE_SLOTS = 2
 # the textbook count for one slice at a time).
 UNBATCHED_FAN_SLOTS = 28
 
+LSQ_SEED = 20261203
+RAGGED_SEED = 20261204
+PRECISION_SEED = 20261205
+
 CFG = dict(nbatch=16, n=512, dx=0.7, na=720, ns=736, ds=1.30, dso=595.0, dsd=1085.6,
            i0=1e5, ellipses=10, seed=20261202)
 
@@ -113,6 +117,112 @@ def cpu_baseline(cfg, sinos, threads, budget_s=30.0):
                 pixel_views=pv, ref_f32=outs[0], ref_f64=outs[1])
 
 
+def bench_lsq(torch, steps):
+    """Least squares min 0.5||FP x - y||^2 on 8 seeded-ellipse 512^2 slices,
+    parallel beam, 720 views over 180 deg, 729 channels: `steps` gradient
+    steps x <- x - FP^T(FP x - y)/||FP||^2 through autograd, timed."""
+    from ct_recon_extensions_amd import phantom as P
+    from ct_recon_extensions_amd import recon
+    scan = recon.ParallelScan()
+    fp, fpt = recon.projector(scan)
+    truth = np.stack([P.rasterize(P.random_ellipses(10, LSQ_SEED + b, radius=scan.n / 2),
+                                  scan.n, scan.n, scan.dx, scan.dx) for b in range(8)])
+    xt = torch.from_numpy(truth.astype(np.float32)).cuda()
+    y = fp(xt)
+    lam = recon.operator_norm_sq(scan, "cuda", iters=20)
+    g = torch.Generator(device="cuda").manual_seed(LSQ_SEED)
+    xa = torch.randn(xt.shape, generator=g, device="cuda")
+    ya = torch.randn(y.shape, generator=g, device="cuda")
+    ax, aty = fp(xa), fpt(ya)
+    gap = abs(float((ax.double() * ya.double()).sum()) - float((xa.double() * aty.double()).sum()))
+    gap /= float(ax.double().norm()) * float(ya.double().norm())
+    recon.gradient_descent(torch.zeros_like(xt), y, scan, steps=3, alpha=1.0 / lam)  # warm-up
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    x, losses = recon.gradient_descent(torch.zeros_like(xt), y, scan, steps=steps,
+                                       alpha=1.0 / lam)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    ms_fp = time_events(lambda: fp(x), 10, torch)
+    ms_fpt = time_events(lambda: fpt(y), 10, torch)
+    ls = losses.cpu().numpy()
+    return {
+        "workload": "8 x 512^2 seeded ellipses, parallel 720 views/180deg, 729 ch, ds=dx=1; "
+                    "x <- x - FP^T(FP x - y)/||FP||^2 via autograd",
+        "steps": steps, "ms_per_grad_step": round(ms, 4), "fp_ms": round(ms_fp, 4),
+        "fpt_ms": round(ms_fpt, 4), "adjoint_gap": gap,
+        "loss_first": float(ls[0]), "loss_last": float(ls[-1]),
+        "rel_err_last": float((x - xt).norm() / xt.norm()),
+        "fp_ray_samples_per_s": 8 * scan.na * scan.ns * scan.n / (ms_fp * 1e-3),
+        "fpt_pixel_views_per_s": 8 * scan.na * scan.n ** 2 / (ms_fpt * 1e-3),
+    }
+
+
+def bench_ragged(torch, reps):
+    """64 seeded fan problems (n in {128,256,384,512}, 180..1440 irregular
+    views, 256..1024 channels, SID/SDD 0.5..0.6) backprojected in one
+    persistent launch replayed from a HIP graph; tile and view chunk from the
+    autotuner."""
+    from ct_recon_extensions_amd import ragged as R
+    params, angles = R.random_fan_problems(64, seed=RAGGED_SEED)
+    rb = R.RaggedFanBackprojection2DAngle(params, angles, autotune=True, tune_reps=3)
+    rb.proj_flat.normal_(generator=torch.Generator(device="cuda").manual_seed(RAGGED_SEED))
+    rb(graph=True)
+    torch.cuda.synchronize()
+    ms = time_events(lambda: rb(graph=True), reps, torch)
+    ms_eager = time_events(lambda: rb(graph=False), reps, torch)
+    busy = rb.balance()
+    out = {
+        "workload": "64 fan problems, bp_fan_2d_angle, one persistent graph-replayed launch",
+        "seed": RAGGED_SEED, "pixel_views": rb.pixel_views, "ms_per_batch": round(ms, 4),
+        "ms_per_batch_eager": round(ms_eager, 4),
+        "gpixel_views_per_s": round(rb.pixel_views / (ms * 1e-3) / 1e9, 2),
+        "tile": R.TILES[rb.tile], "vchunk": rb.vchunk, "autotune_ms": rb.tune_table,
+        "work_items": rb.nitems, "grid": rb.grid, "busy_fraction": round(busy, 4),
+        "unbatched_28slot_model_frac": round(
+            rb.pixel_views / (ms * 1e-3) * UNBATCHED_FAN_SLOTS / PEAK_VALU_SLOTS, 4),
+    }
+    rb.close()
+    return out
+
+
+def bench_filter_precision(torch, reps):
+    """Fan-weighted Ram-Lak filter on the 32-slice precision workload by the
+    BF16x3 MFMA Toeplitz GEMM, the FP32 MFMA GEMM and the direct FP32 kernel,
+    each against the FP64 filter per slice (relative L2), with times and the
+    matrix-core roofline of the GEMMs."""
+    import ct_recon_extensions_amd as F
+    from ct_recon_extensions_amd import phantom as P
+    x, kw, i0s = P.shepp_logan_fan_batch(32, PRECISION_SEED, "cuda")
+    ref = F.fan_filter(x.double(), **kw)
+    nb, na, ns = x.shape
+    res = {"workload": "32 x 1440 x 1472 Shepp-Logan-type fan sinograms, Poisson I0 1e4..1e6, "
+                       "fan weighting + Ram-Lak", "seed": PRECISION_SEED,
+           "i0": [round(v) for v in i0s], "ms": {}, "rel_l2_per_slice": {},
+           "worst_rel_l2": {}}
+    for impl in ("bf16x3", "mfma", "direct"):
+        y = F.fan_filter(x, **kw, impl=impl)
+        d = (y.double() - ref).reshape(nb, -1).norm(dim=1) / ref.reshape(nb, -1).norm(dim=1)
+        errs = [float(v) for v in d.cpu()]
+        res["rel_l2_per_slice"][impl] = [float(f"{e:.3e}") for e in errs]
+        res["worst_rel_l2"][impl] = max(errs)
+        res["ms"][impl] = round(time_events(lambda: F.fan_filter(x, **kw, impl=impl), reps,
+                                            torch), 4)
+    # Parity-split Toeplitz GEMM: M = rows, N = K = ns, half the dense MACs.
+    macs = nb * na * ns * ns / 2
+    res["roofline_bf16x3"] = {"bound": "mfma", "flops": 3 * 2 * macs,
+                              "achieved_tflops": round(3 * 2 * macs / (res["ms"]["bf16x3"] * 1e9), 2),
+                              "peak_tflops": 2500.0}
+    res["roofline_fp32_mfma"] = {"bound": "mfma", "flops": 2 * macs,
+                                 "achieved_tflops": round(2 * macs / (res["ms"]["mfma"] * 1e9), 2),
+                                 "peak_tflops": 157.3}
+    res["hbm_bytes"] = 2 * x.numel() * 4
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +232,10 @@ def main():
                     help="host threads for the CPU baseline (default: min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-slice GPU path")
+    ap.add_argument("--no-lsq", action="store_true", help="skip the least-squares grad steps")
+    ap.add_argument("--lsq-steps", type=int, default=200)
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batch")
+    ap.add_argument("--no-precision", action="store_true", help="skip the filter precision study")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path shards by slice with no exchange, so "
@@ -233,6 +347,20 @@ def main():
         },
         "plain_per_slice_gpu": plain,
     }
+    if not args.no_lsq:
+        result["lsq_grad_step"] = bench_lsq(torch, args.lsq_steps)
+        print(f"[bench] lsq grad step {result['lsq_grad_step']['ms_per_grad_step']:.3f} ms",
+              file=sys.stderr, flush=True)
+    if not args.no_ragged:
+        result["ragged"] = bench_ragged(torch, args.steps)
+        print(f"[bench] ragged {result['ragged']['ms_per_batch']:.3f} ms", file=sys.stderr,
+              flush=True)
+
+    if not args.no_precision:
+        result["filter_precision"] = bench_filter_precision(torch, args.steps)
+        fp_ = result["filter_precision"]
+        print(f"[bench] filter bf16x3 {fp_['ms']['bf16x3']:.3f} ms, worst slice rel L2 "
+              f"{fp_['worst_rel_l2']['bf16x3']:.2e}", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -30,6 +30,17 @@ class CtrGeom(C.Structure):
     ]
 
 
+class CtrRaggedProblem(C.Structure):
+    """Mirror of ``ctr_ragged_problem`` (include/ctr_api.h)."""
+
+    _fields_ = [
+        ("g", CtrGeom),
+        ("xpos", C.c_void_p), ("ypos", C.c_void_p), ("sincostbl", C.c_void_p),
+        ("proj", C.c_void_p), ("img", C.c_void_p),
+        ("scale", C.c_double),
+    ]
+
+
 class CtrError(RuntimeError):
     pass
 
@@ -59,13 +70,17 @@ _SIGS = {
     "ctr_ramp": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, _P]),
     "ctr_fan_weighting": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P]),
     "ctr_ramp_toeplitz": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, C.c_int, _P]),
+    "ctr_ragged_fan_bp_plan": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
+    "ctr_ragged_fan_bp_run": (C.c_int, [_P, C.c_int, _P]),
+    "ctr_ragged_fan_bp_info": (C.c_int, [_P, _I, _I, _I, _I, _D]),
+    "ctr_ragged_fan_bp_balance": (C.c_int, [_P, _P, _D]),
+    "ctr_ragged_fan_bp_destroy": (C.c_int, [_P]),
+    "ctr_ragged_fan_bp_autotune": (C.c_int, [_P, C.c_int, C.c_int, _P, _I, _I,
+                                             C.POINTER(C.c_float)]),
 }
 
 # Entry points that only exist once the corresponding sources are built.
-_OPTIONAL = {
-    "ctr_fan_bp_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
-    "ctr_ragged_fan_bp": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_int, _P]),
-}
+_OPTIONAL: dict = {}
 
 _lock = threading.Lock()
 _lib = None

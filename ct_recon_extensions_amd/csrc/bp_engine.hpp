@@ -100,24 +100,26 @@ __device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
     w0 = w - w1;
 }
 
+// One pixel tile x NB slices (starting at slice b0) over all views: the
+// whole forward computation of one workgroup.  smem holds kAnchorChunk
+// anchors followed by two span buffers of vchunk x NB x LROW floats.  The
+// first LDS write comes after a barrier, so a persistent caller may run
+// tiles back to back.
 template <class Geo, int TW, int TH, int NB, int LROW>
-__global__ __launch_bounds__((TileShape<TW, TH>::kBlock))
-__attribute__((amdgpu_waves_per_eu(4, 4))) void bp_forward_kernel(
-    const typename Geo::Tables T, const BpLaunch<Geo> L)
+__device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpLaunch<Geo>& L,
+                                        int tile, int b0, unsigned char* smem)
 {
     using TS = TileShape<TW, TH>;
     using Anchor = typename Geo::Anchor;
     constexpr int BLOCK = TS::kBlock, PPT = TS::kPPT, NWAVES = BLOCK / 64;
     static_assert(LROW % 64 == 0, "span rows are whole waves of dwords");
-    extern __shared__ __align__(16) unsigned char smem[];
     Anchor* anchors = reinterpret_cast<Anchor*>(smem);
     float* spans = reinterpret_cast<float*>(smem + kAnchorChunk * sizeof(Anchor));
     const int bufsz = L.vchunk * NB * LROW;  // floats per buffer
 
     const auto& P = L.P;
     const int tid = threadIdx.x, lane = tid & 63, wave = uniform_i(tid >> 6);
-    const TileFrame F(blockIdx.x, L.tiles_x, P.nx, P.ny, TW, TH);
-    const int b0 = blockIdx.y * NB;
+    const TileFrame F(tile, L.tiles_x, P.nx, P.ny, TW, TH);
     const int nb = min(NB, L.nbatch - b0);
     const int ilo = F.x0 - F.ic, ihi = F.x0 + TW - 1 - F.ic;
     const int jlo = F.y0 - F.jc, jhi = F.y0 + TH - 1 - F.jc;
@@ -228,6 +230,19 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) void bp_forward_kernel(
                         static_cast<float>(static_cast<double>(total[k][b]) * L.scale);
         }
     }
+}
+
+template <class Geo, int TW, int TH, int NB, int LROW>
+__global__ __launch_bounds__((TileShape<TW, TH>::kBlock))
+__attribute__((amdgpu_waves_per_eu(4, 4))) void bp_forward_kernel(
+    const typename Geo::Tables T, const BpLaunch<Geo> L)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    // XCD-aware order of the tiles when the grid is one slice group (the
+    // linear id then equals blockIdx.x).
+    const int ntiles = L.tiles_x * L.tiles_y;
+    const int tile = gridDim.y == 1 ? xcd_tile(blockIdx.x, ntiles) : blockIdx.x;
+    bp_tile<Geo, TW, TH, NB, LROW>(T, L, tile, blockIdx.y * NB, smem);
 }
 
 // View-parallel scatter transpose.  One workgroup owns the detector rows of

@@ -80,7 +80,6 @@ __global__ void fan_bpt_f64_kernel(const double* __restrict__ img, double* __res
 }
 
 // ---------------------------------------------------------------- host side
-namespace {
 
 // Smallest source-to-pixel distance over the tile-padded pixel grid (every
 // source position sits at distance dso from the rotation centre).
@@ -133,6 +132,8 @@ int fan_problem(const ctr_geom& g, int TW, int TH, FanProblem& P, int& lmax, boo
     exact = (k * std::pow(umax, 7) / 7 > 2e-7) || (std::pow(umax, 6) > 5e-8) || umax > 0.25;
     return CTR_OK;
 }
+
+namespace {
 
 template <int TW, int TH, int NB, bool EXACT, int LROW>
 int launch_fan_bp_l(BpLaunch<FanGeo<EXACT>> L, const FanTables& T, hipStream_t st)

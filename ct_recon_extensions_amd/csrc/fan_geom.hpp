@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "ctr_common.hpp"
 #include "tile_common.hpp"
 
 namespace ctr {
@@ -121,6 +122,12 @@ __device__ __forceinline__ void fan_pixel(const FanProblem& P, const FanAnchor& 
         w = rd * rd * fmaf(v2, fmaf(v2, P.g2, P.g1), 1.0f);
     }
 }
+
+// Host side (fan_bp.hip): geometry checks for a TW x TH tiling, the
+// channels one tile's span needs (lmax) and whether the series is accurate
+// enough (exact = false) -- and the smallest source-to-pixel distance.
+int fan_problem(const ctr_geom& g, int TW, int TH, FanProblem& P, int& lmax, bool& exact);
+double fan_min_distance(const ctr_geom& g, int TW, int TH);
 
 // Policy for the BP engine.
 template <bool EXACT>

@@ -18,6 +18,21 @@ struct TileShape {
     static_assert(kPix % kBlock == 0, "tile must be a multiple of the block");
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id, and
+// each XCD has its own L2.  Mapping the id so that every XCD receives one
+// contiguous band of tiles keeps the detector spans a band needs within one
+// L2 (instead of every XCD reading every sinogram row).  Bijective for any
+// tile count; ids >= ntiles (none in practice) map to themselves.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_tile(int id, int ntiles)
+{
+    if (id >= ntiles) return id;
+    const int q = ntiles / kXcds, r = ntiles % kXcds;
+    const int x = id % kXcds, k = id / kXcds;
+    return x * q + min(x, r) + k;
+}
+
 // Origin of a tile and the pixel (ic, jc) its anchor is evaluated at.
 struct TileFrame {
     int x0, y0, ic, jc;

@@ -159,6 +159,50 @@ int ctr_plain_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, 
                      const double* xpos, const double* ypos, const double* sincostbl,
                      double scale, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Ragged batches of FanBackprojection2DAnglePixDriven<float>
+ * (src/bp_fan_2d_angle.cc:47-91): problems with different image sizes,
+ * view counts, detectors and source distances in one persistent launch
+ * (replaces the reference's serial per-slice loop over one geometry,
+ * tensorflow/bp_fan_2d_ops.h:107-128).  A plan fixes the problems (their
+ * geometry, device tables and float32 buffers), the pixel tile (0 16x16,
+ * 1 8x8, 2 32x8) and the views per LDS round (1..64); running it clears a
+ * work counter with a kernel and launches the persistent kernel, either
+ * directly or (use_graph != 0) by replaying a HIP graph captured on the
+ * first such call.  img = scale * sum over views, scale = 1 for the
+ * reference's bp_fan_2d_angle.  Runs of one plan must not overlap.
+ * ------------------------------------------------------------------- */
+typedef struct ctr_ragged_problem {
+    ctr_geom g;                 /* ns, na, ds, offset_s, nx, ny, dx, dy, offsets, dso, dsd */
+    const double* xpos;         /* device tables of ctr_fan_angle_tables */
+    const double* ypos;
+    const double* sincostbl;
+    const float* proj;          /* device [na][ns] */
+    float* img;                 /* device [ny][nx] */
+    double scale;
+} ctr_ragged_problem;
+
+typedef struct ctr_ragged_plan ctr_ragged_plan;
+
+int ctr_ragged_fan_bp_plan(const ctr_ragged_problem* probs, int nprob, int tile, int vchunk,
+                           ctr_ragged_plan** plan);
+int ctr_ragged_fan_bp_run(ctr_ragged_plan* plan, int use_graph, void* stream);
+/* Work items (pixel tiles), persistent grid, block, LDS bytes and the
+ * pixel*view count of a plan. */
+int ctr_ragged_fan_bp_info(const ctr_ragged_plan* plan, int* nitems, int* grid, int* block,
+                           int* lds_bytes, double* pixel_views);
+/* One run that also records every workgroup's first and last clock tick;
+ * busy_fraction = sum of workgroup busy time / (grid x span).  Synchronises
+ * the stream. */
+int ctr_ragged_fan_bp_balance(ctr_ragged_plan* plan, void* stream, double* busy_fraction);
+int ctr_ragged_fan_bp_destroy(ctr_ragged_plan* plan);
+/* Time every tile {16x16, 8x8, 32x8} x view chunk {8, 16, 32, 64} over
+ * `reps` runs (after one warm-up) and return the fastest; ms_table
+ * (optional, 12 floats, tile-major) receives each mean, -1 where the
+ * geometry does not admit the configuration.  Synchronises the stream. */
+int ctr_ragged_fan_bp_autotune(const ctr_ragged_problem* probs, int nprob, int reps,
+                               void* stream, int* best_tile, int* best_vchunk, float* ms_table);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,108 @@
+"""Least-squares reconstruction on the differentiable ray-driven projector.
+
+At the full workload (8 x 512^2, 720 views over 180 deg, 729 channels):
+dot-product adjoint gap <= 1e-6 on seeded Gaussian x, y; central finite
+differences (eps = 1e-3 max|x|) of 0.5 ||FP x - y||^2 against the autograd
+gradient on 16 coordinates (relative 1e-3 in float32, 1e-7 in float64);
+the loss decreases monotonically over gradient steps.  On a small scan the
+GPU iteration is compared with the same iteration run on the oracle's FP
+and transpose in float64.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from ct_recon_extensions_amd import _lib, recon
+    _lib.load()
+    return torch, recon
+
+
+def test_adjoint_full_size(env):
+    torch, recon = env
+    scan = recon.ParallelScan()
+    fp, fpt = recon.projector(scan)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.randn((8, scan.n, scan.n), generator=g, device="cuda")
+    y = torch.randn((8, scan.na, scan.ns), generator=g, device="cuda")
+    ax, aty = fp(x), fpt(y)
+    lhs = float((ax.double() * y.double()).sum())
+    rhs = float((x.double() * aty.double()).sum())
+    gap = abs(lhs - rhs) / (float(ax.double().norm()) * float(y.double().norm()))
+    assert gap < 1e-6, gap
+
+
+@pytest.mark.parametrize("dtype,tol", [("float32", 1e-3), ("float64", 1e-7)])
+def test_finite_differences_full_size(env, dtype, tol):
+    torch, recon = env
+    dt = getattr(torch, dtype)
+    scan = recon.ParallelScan()
+    fp, _ = recon.projector(scan)
+    g = torch.Generator(device="cuda").manual_seed(4)
+    nb = 8 if dtype == "float32" else 2
+    x = torch.randn((nb, scan.n, scan.n), generator=g, device="cuda").to(dt)
+    y = fp(torch.randn((nb, scan.n, scan.n), generator=g, device="cuda").to(dt))
+    _, grad = recon.loss_and_grad(x, y, fp)
+    eps = 1e-3 * float(x.abs().max())
+    rng = np.random.default_rng(5)
+    for _ in range(16):
+        b, i, j = int(rng.integers(nb)), int(rng.integers(scan.n)), int(rng.integers(scan.n))
+        f = []
+        for sgn in (1, -1):
+            xp = x.clone()
+            xp[b, i, j] += sgn * eps
+            r = (fp(xp) - y).double()
+            f.append(0.5 * float((r * r).sum()))
+        fd = (f[0] - f[1]) / (2 * eps)
+        an = float(grad[b, i, j])
+        assert abs(fd - an) <= tol * max(abs(an), 1e-3 * float(grad.abs().max())), (b, i, j, fd, an)
+
+
+def test_gradient_descent_decreases_loss_full_size(env):
+    torch, recon = env
+    scan = recon.ParallelScan()
+    fp, _ = recon.projector(scan)
+    from ct_recon_extensions_amd import phantom as P
+    truth = np.stack([P.rasterize(P.random_ellipses(10, 30 + b, radius=scan.n / 2), scan.n,
+                                  scan.n, scan.dx, scan.dx) for b in range(8)]).astype(np.float32)
+    xt = torch.from_numpy(truth).cuda()
+    y = fp(xt)
+    lam = recon.operator_norm_sq(scan, "cuda", iters=20)
+    x, losses = recon.gradient_descent(torch.zeros_like(xt), y, scan, steps=20, alpha=1.0 / lam)
+    ls = losses.cpu().numpy()
+    assert np.all(np.diff(ls) < 0), ls
+    assert ls[-1] < 0.5 * ls[0]
+    err0 = float(xt.norm())
+    assert float((x - xt).norm()) < err0
+
+
+def test_gradient_descent_matches_oracle_iteration(env):
+    torch, recon = env
+    scan = recon.ParallelScan(n=48, na=60, ns=69)
+    fp, _ = recon.projector(scan)
+    g = O.geom(ns=scan.ns, na=scan.na, ds=scan.ds, orbit=math.pi / scan.na, nx=scan.n,
+               ny=scan.n, dx=scan.dx, dy=scan.dx)
+    t_fp, t_gr = O.fp_prep(g), O.fp_grad_prep(g)
+    rng = np.random.default_rng(9)
+    truth = rng.standard_normal((scan.n, scan.n))
+    y = O.fp(g, truth, t_fp)
+    lam = recon.operator_norm_sq(scan, "cuda", iters=30)
+    alpha = 1.0 / lam
+    xo = np.zeros_like(truth)
+    for _ in range(15):
+        xo = xo - alpha * O.fp_grad(g, O.fp(g, xo, t_fp) - y, t_gr)
+    x, _ = recon.gradient_descent(torch.zeros((1, scan.n, scan.n), device="cuda"),
+                                  torch.from_numpy(y[None].astype(np.float32)).cuda(), scan,
+                                  steps=15, alpha=alpha)
+    assert rel_l2(x[0].cpu().numpy(), xo) < 1e-5
