@@ -117,6 +117,42 @@ def cpu_baseline(cfg, sinos, threads, budget_s=30.0):
                 pixel_views=pv, ref_f32=outs[0], ref_f64=outs[1])
 
 
+TRAFFIC_PROFILE = "profiles/fan_bp_16x512sq_720v.md"
+
+
+def profiled_traffic():
+    """HBM bytes per BP launch recorded by the counter runs of --profile-bp."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def profile_bp(torch, F):
+    """Fan BP only (for rocprofv3 --pmc): three launches on random sinograms of
+    the workload's shape, then a copy of a 256 MiB buffer that was evicted
+    from L2 and MALL by writing 1 GiB elsewhere: its FETCH_SIZE / WRITE_SIZE
+    against 256 MiB calibrate the counters."""
+    cfg = CFG
+    g = torch.Generator(device="cuda").manual_seed(0)
+    proj = torch.randn((cfg["nbatch"], cfg["na"], cfg["ns"]), generator=g, device="cuda")
+    for _ in range(3):
+        img = F.fan_backprojection_2d(proj, **kw_fan(cfg))
+    torch.cuda.synchronize()
+    src = torch.randn(64 * 1024 * 1024, generator=g, device="cuda")
+    evict = torch.zeros(256 * 1024 * 1024, device="cuda")
+    torch.cuda.synchronize()
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": "fan_bp", "checksum": float(img.double().abs().sum()),
+                      "copy_bytes": src.numel() * 4, "evict_bytes": evict.numel() * 4}))
+
+
 def bench_lsq(torch, steps):
     """Least squares min 0.5||FP x - y||^2 on 8 seeded-ellipse 512^2 slices,
     parallel beam, 720 views over 180 deg, 729 channels: `steps` gradient
@@ -236,6 +272,9 @@ def main():
     ap.add_argument("--lsq-steps", type=int, default=200)
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batch")
     ap.add_argument("--no-precision", action="store_true", help="skip the filter precision study")
+    ap.add_argument("--profile-bp", action="store_true",
+                    help="counter-collection mode: three fan BP launches of the workload, then a "
+                         "256 MiB device copy from a cold source (calibrates FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path shards by slice with no exchange, so "
@@ -247,6 +286,10 @@ def main():
     import ct_recon_extensions_amd as F
     from ct_recon_extensions_amd import _lib
     from ct_recon_extensions_amd.geometry import FanBackprojection2DParam, device_tables
+
+    if args.profile_bp:
+        profile_bp(torch, F)
+        return
 
     lib = _lib.load()
     cfg = dict(CFG)
@@ -338,7 +381,9 @@ def main():
             "unit": "Tslot/s (FP32 VALU lane-issues)",
             "frac": round(valu_achieved / PEAK_VALU_SLOTS, 4),
             "slots_per_launch": slots,
-            "traffic": None,
+            "traffic": profiled_traffic(),
+            "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                            "scaled by a cold 256 MiB copy (" + TRAFFIC_PROFILE + ")",
             "hbm": {"achieved": round(hbm_achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(hbm_achieved / PEAK_HBM_GBS, 5),
                     "algorithmic_bytes": hbm_bytes},

@@ -83,21 +83,39 @@ __device__ __forceinline__ void stage_spans(const BpLaunch<Geo>& L,
 }
 
 // Geometry of one pixel-view in the form the accumulation needs: LDS word
-// index of the left tap and the two tap weights.
+// index of the left tap and the two tap weights.  Interpolating policies
+// (the default) give a channel position and a weight and get the linear
+// interpolation taps, zeroed off the detector; a policy with
+// kCustomTaps = true computes its own two taps (Geo::taps) and relies on the
+// zero padding of the staged span outside [0, ns).
+template <class Geo>
+struct HasCustomTaps {
+    template <class G>
+    static constexpr bool test(decltype(G::kCustomTaps)*) { return G::kCustomTaps; }
+    template <class G>
+    static constexpr bool test(...) { return false; }
+    static constexpr bool value = test<Geo>(nullptr);
+};
+
 template <class Geo, int LROW>
 __device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
                                             const typename Geo::Anchor& A, int inside, int is0,
                                             float fi, float fj, int& idx, float& w0, float& w1)
 {
-    float f, w;
-    Geo::pixel(P, A, fi, fj, f, w);
-    const float fl = floorf(f);
-    const float fr = f - fl;
-    idx = static_cast<int>(fl);
-    if (__builtin_expect(!inside, 0)) w = on_detector(is0 + idx, fr, P.ns) ? w : 0.0f;
-    idx = min(max(idx, 0), LROW - 2);
-    w1 = w * fr;
-    w0 = w - w1;
+    if constexpr (HasCustomTaps<Geo>::value) {
+        Geo::taps(P, A, fi, fj, idx, w0, w1);
+        idx = min(max(idx, 0), LROW - 2);
+    } else {
+        float f, w;
+        Geo::pixel(P, A, fi, fj, f, w);
+        const float fl = floorf(f);
+        const float fr = f - fl;
+        idx = static_cast<int>(fl);
+        if (__builtin_expect(!inside, 0)) w = on_detector(is0 + idx, fr, P.ns) ? w : 0.0f;
+        idx = min(max(idx, 0), LROW - 2);
+        w1 = w * fr;
+        w0 = w - w1;
+    }
 }
 
 // One pixel tile x NB slices (starting at slice b0) over all views: the

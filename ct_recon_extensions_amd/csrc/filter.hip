@@ -10,7 +10,10 @@
 // other than the centre are zero (src/filter.cc:26-30, :45-49), which halves
 // the work.  The tap index is uniform across the workgroup (scalar loads),
 // each tap is applied to all ROWS rows, and every lane reads consecutive
-// LDS words (no bank conflicts).  FP32 products, FP32 sums.
+// LDS words (no bank conflicts).  The centre tap and the odd taps |m| <= 15
+// (~97% of the filter's weight; on smooth rows they nearly cancel) are
+// accumulated in FP64 with FP64 taps; the far field in the row type (FP32
+// for float rows), added in FP64 at the end.
 // The transpose (RampFilterGrad) is the same operator since h is symmetric.
 #include "ctr_common.hpp"
 
@@ -35,9 +38,12 @@ __device__ inline double cos_weight(const RampParams& P, int is)
 }
 
 // hf: taps in the accumulation type, hf[ns + m] = h[m]; zero-padded rows in LDS.
+constexpr int kRampNear = 15;  // |m| <= kRampNear: FP64 near field
+
 template <typename T, typename Acc = typename std::conditional<sizeof(T) == 8, double, float>::type>
 __global__ __launch_bounds__(256) void ramp_kernel(const T* __restrict__ in, T* __restrict__ out,
-                                                   const Acc* __restrict__ hf, RampParams P)
+                                                   const Acc* __restrict__ hf,
+                                                   const double* __restrict__ h64, RampParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     Acc* xs = reinterpret_cast<Acc*>(smem);  // [ROWS][3*ns]
@@ -55,14 +61,32 @@ __global__ __launch_bounds__(256) void ramp_kernel(const T* __restrict__ in, T* 
         xs[e] = v;
     }
     __syncthreads();
-    const Acc h0 = hf[ns];
+    const double h0 = h64[ns];
+    // odd m in [-(ns-1), ns-1]; x index ns + is - m stays in [1, 3ns-1]
+    const int mlo = -(ns - 1) | 1;  // smallest odd >= -(ns-1)
+    const int nlo = max(mlo, -kRampNear), nhi = min(ns - 1, kRampNear);
     for (int is = threadIdx.x; is < ns; is += 256) {
         Acc acc[kRampRows];
+        double near[kRampRows];
 #pragma unroll
-        for (int r = 0; r < kRampRows; ++r) acc[r] = h0 * xs[r * pitch + ns + is];
-        // odd m in [-(ns-1), ns-1]; x index ns + is - m stays in [1, 3ns-1]
-        const int mlo = -(ns - 1) | 1;  // smallest odd >= -(ns-1)
-        for (int m = mlo; m <= ns - 1; m += 2) {
+        for (int r = 0; r < kRampRows; ++r) {
+            acc[r] = Acc(0);
+            near[r] = h0 * static_cast<double>(xs[r * pitch + ns + is]);
+        }
+        for (int m = mlo; m < nlo; m += 2) {
+            const Acc h = hf[ns + m];
+            const int xi = ns + is - m;
+#pragma unroll
+            for (int r = 0; r < kRampRows; ++r) acc[r] = fma(h, xs[r * pitch + xi], acc[r]);
+        }
+        for (int m = nlo; m <= nhi; m += 2) {
+            const double h = h64[ns + m];
+            const int xi = ns + is - m;
+#pragma unroll
+            for (int r = 0; r < kRampRows; ++r)
+                near[r] = fma(h, static_cast<double>(xs[r * pitch + xi]), near[r]);
+        }
+        for (int m = nhi + 2; m <= ns - 1; m += 2) {
             const Acc h = hf[ns + m];
             const int xi = ns + is - m;
 #pragma unroll
@@ -72,7 +96,7 @@ __global__ __launch_bounds__(256) void ramp_kernel(const T* __restrict__ in, T* 
         for (int r = 0; r < kRampRows; ++r)
             if (r < nr)
                 out[static_cast<size_t>(r0 + r) * ns + is] =
-                    static_cast<T>(static_cast<double>(acc[r]) * P.ds);
+                    static_cast<T>((near[r] + static_cast<double>(acc[r])) * P.ds);
     }
 }
 
@@ -118,7 +142,7 @@ int ramp_typed(const ctr_geom& g, const T* in, T* out, unsigned int nrow, const 
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         static_cast<int>(lds)));
         hipLaunchKernelGGL(kern, dim3(div_up(nrow, kRampRows)), dim3(256), lds, st, in, out,
-                           static_cast<const Acc*>(hf), P);
+                           static_cast<const Acc*>(hf), filter, P);
         rc = check_launch("ramp_kernel");
     }
     CTR_HIP(hipFreeAsync(hf, st));

@@ -16,7 +16,18 @@
 // channels whose rays pass within one pixel step are gathered with the
 // triangle weight 1 - |mid - is|/o1 of the reference's transpose
 // (src/fp_par_2d.cc:355-385), the channel range clamped to the detector.
+//
+// Transpose on the BP engine (FptGeo): the pixel's centre projects to the
+// channel mid = begin + i*o1 + j*o2 (linear in the pixel), and the channels
+// within |o1| of it get 1 - |c - mid|/|o1|; with |o1| <= 1 (pixel pitch along
+// the marched axis at most the channel pitch) those are at most floor(mid)
+// and floor(mid)+1, i.e. two taps, so the engine's LDS-staged spans, slice
+// batching and FP64 anchors apply unchanged.  The first and last pixel
+// along the marched axis keep one side only (src/fp_par_2d.cc:360-368).
+#include "bp_engine.hpp"
 #include "ctr_common.hpp"
+#include "fpt_geom.hpp"
+#include "tile_common.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -100,6 +111,215 @@ __global__ __launch_bounds__(256) void fp_kernel(const T* __restrict__ img,
                 static_cast<T>(static_cast<double>(sum[b]) * dw);
 }
 
+// FP with LDS-staged image rows (float32).  A workgroup owns 256 consecutive
+// channels (one per thread) of V consecutive views and NB slices, and marches
+// the rows of the image (x-branch views) or of its transpose (y-branch
+// views) in chunks of kFpRows: per chunk it stages, for all NB slices, the
+// span of columns its V x 256 rays can touch (DMA from global straight into
+// LDS; zero outside the row), then every thread advances its V rays through
+// those rows.  A staged pixel is read by ~V rays instead of every view
+// re-reading the image through L1/L2.  Each ray keeps the reference's FP64
+// position accumulation (x += offset per row), inclusion test, taps and FP32
+// sum order, so the result is bit-identical to fp_kernel.  Views of the other
+// branch in the group are handled in a second pass.
+constexpr int kFpRows = 4;
+constexpr int kFpSpan = 512;  // staged columns per row (host checks the bound)
+
+template <int NB, int V>
+__global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ img,
+                                                     const float* __restrict__ imgT,
+                                                     float* __restrict__ proj,
+                                                     const double* __restrict__ sincos,
+                                                     const double* __restrict__ bo,
+                                                     const int* __restrict__ usex, FpProblem P)
+{
+    __shared__ __align__(16) float rows[kFpRows * NB * kFpSpan];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int s0 = blockIdx.x * 256, s = s0 + tid;
+    const int slast = min(s0 + 255, P.ns - 1);
+    const int a0 = blockIdx.y * V;
+    const int b0 = blockIdx.z * NB;
+    const int nb = min(NB, P.nbatch - b0);
+    const size_t slice = static_cast<size_t>(P.nx) * P.ny;
+
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool xb = pass == 0;
+        // views of this pass
+        bool act[V];
+        int nact = 0;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int a = a0 + v;
+            act[v] = a < P.na && (usex[a] != 0) == xb;
+            nact += act[v] ? 1 : 0;
+        }
+        if (nact == 0) continue;  // uniform
+        const int nsteps = xb ? P.ny : P.nx;
+        const int rowlen = xb ? P.nx : P.ny;
+        const float* src = xb ? img : imgT;
+        const double hi = rowlen - 1;
+        double x[V], off[V], blo[V], bhi[V];
+        float sum[V][NB];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int a = min(a0 + v, P.na - 1);
+            const double* r = bo + 2 * static_cast<size_t>(a) * P.ns;
+            x[v] = s < P.ns ? r[2 * s] : 0.0;
+            off[v] = r[1];
+            blo[v] = r[2 * s0];
+            bhi[v] = r[2 * slast];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) sum[v][b] = 0.0f;
+        }
+        for (int k0 = 0; k0 < nsteps; k0 += kFpRows) {
+            const int nr = min(kFpRows, nsteps - k0);
+            // columns the chunk's rays can touch (one column of margin each side)
+            double xmin = 1e300, xmax = -1e300;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if (!act[v]) continue;
+                const double d0 = k0 * off[v], d1 = (k0 + nr - 1) * off[v];
+                xmin = fmin(xmin, fmin(blo[v], bhi[v]) + fmin(d0, d1));
+                xmax = fmax(xmax, fmax(blo[v], bhi[v]) + fmax(d0, d1));
+            }
+            __syncthreads();  // previous chunk consumed
+            if (xmax < -1.0 || xmin > hi + 1.0) {
+                // no ray of the chunk is on the image: only advance them
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    for (int k = 0; k < nr; ++k) x[v] += off[v];
+                continue;
+            }
+            const int cmin = max(0, static_cast<int>(floor(xmin)) - 1);
+            const int cmax = min(rowlen - 1, static_cast<int>(floor(xmax)) + 2);
+            // 64-column DMA rows; the host keeps cmax - cmin < kFpSpan
+            const int nload = min((cmax - cmin + 64) / 64, kFpSpan / 64);
+            // stage rows (k0 + kk, slice b) -> rows[(kk * NB + b) * kFpSpan + c]
+            for (int e = wave; e < nr * NB; e += 4) {
+                const int kk = e / NB, b = e - kk * NB;
+                const float* rowp = src + static_cast<size_t>(b0 + b) * slice +
+                                    static_cast<size_t>(k0 + kk) * rowlen;
+                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<float*>(rowp), 0,
+                    b < nb ? static_cast<unsigned int>(rowlen) * 4u : 0u, 0x00020000);
+                float* dst = rows + e * kFpSpan;
+                for (int c = 0; c < nload; ++c)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rs, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4,
+                        (cmin + 64 * c + lane) * 4, 0, 0, 0);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (s < P.ns) {
+#pragma unroll 1
+                for (int kk = 0; kk < nr; ++kk) {
+                    const float* rk = rows + kk * NB * kFpSpan - cmin;
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        if (act[v] && x[v] >= 0 && x[v] <= hi) {
+                            const double fl = floor(x[v]);
+                            const int i1 = static_cast<int>(fl);
+                            const int i2 = min(i1 + 1, rowlen - 1);
+                            const float u = static_cast<float>(x[v] - fl);
+#pragma unroll
+                            for (int b = 0; b < NB; ++b) {
+                                const float p0 = rk[b * kFpSpan + i1];
+                                const float p1 = rk[b * kFpSpan + i2];
+                                sum[v][b] += p0 + u * (p1 - p0);
+                            }
+                        }
+                        x[v] += off[v];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    for (int k = 0; k < nr; ++k) x[v] += off[v];
+            }
+        }
+        if (s < P.ns) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if (!act[v]) continue;
+                const int a = a0 + v;
+                const double dw = xb ? fabs(P.dy / sincos[2 * a + 1]) : fabs(P.dx / sincos[2 * a]);
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+                    if (b < nb)
+                        proj[(static_cast<size_t>(b0 + b) * P.na + a) * P.ns + s] =
+                            static_cast<float>(static_cast<double>(sum[v][b]) * dw);
+            }
+        }
+    }
+}
+
+// Pixels of the image border (first / last row and column), i < 2nx + 2(ny-2).
+__device__ __forceinline__ void border_pixel(int i, int nx, int ny, int& ix, int& iy)
+{
+    if (i < nx) {
+        ix = i;
+        iy = 0;
+    } else if (i < 2 * nx) {
+        ix = i - nx;
+        iy = ny - 1;
+    } else {
+        const int j = i - 2 * nx;
+        ix = j < ny - 2 ? 0 : nx - 1;
+        iy = 1 + (j < ny - 2 ? j : j - (ny - 2));
+    }
+}
+
+// Contribution of view a to pixel (ix, iy) for slices b0 .. b0+nb-1,
+// added to sum[]: the reference's gather (src/fp_par_2d.cc:355-385).
+template <typename T, int NB>
+__device__ __forceinline__ void fpt_view(const T* __restrict__ proj,
+                                         const double* __restrict__ weights,
+                                         const double* __restrict__ pos,
+                                         const int* __restrict__ usex, const FpProblem& P,
+                                         int stride, int a, int ix, int iy, int b0, int nb,
+                                         double (&sum)[NB])
+{
+    using Acc = typename std::conditional<sizeof(T) == 8, double, float>::type;
+    const size_t pslice = static_cast<size_t>(P.na) * P.ns;
+    double mid, left, right, o1;
+    // mid exactly as the reference rounds it (a product, then a sum): the
+    // one-sided border rule below is decided on it
+    if (usex[a]) {
+        const double* pp = pos + 2 * (static_cast<size_t>(a) * stride + iy);
+        o1 = pp[1];
+        mid = __dadd_rn(pp[0], __dmul_rn(o1, static_cast<double>(ix)));
+        left = ix == 0 ? mid : mid - o1;
+        right = ix == P.nx - 1 ? mid : mid + o1;
+    } else {
+        const double* pp = pos + 2 * (static_cast<size_t>(a) * stride + ix);
+        o1 = pp[1];
+        mid = __dadd_rn(pp[0], __dmul_rn(o1, static_cast<double>(iy)));
+        left = iy == 0 ? mid : mid - o1;
+        right = iy == P.ny - 1 ? mid : mid + o1;
+    }
+    if (left > right) {
+        const double t = left;
+        left = right;
+        right = t;
+    }
+    const int c0 = max(static_cast<int>(ceil(left)), 0);
+    const int c1 = min(static_cast<int>(floor(right)), P.ns - 1);
+    Acc tmp[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) tmp[b] = Acc(0);
+    for (int c = c0; c <= c1; ++c) {
+        const Acc w = static_cast<Acc>(1.0 - fabs((mid - c) / o1));
+        const T* pr = proj + static_cast<size_t>(a) * P.ns + c;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (b < nb) tmp[b] += w * static_cast<Acc>(pr[(b0 + b) * pslice]);
+    }
+    const double wa = weights[a];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) sum[b] += static_cast<double>(tmp[b]) * wa;
+}
+
+// Every pixel, one thread per pixel for NB slices.
 template <typename T, int NB>
 __global__ __launch_bounds__(256) void fpt_kernel(const T* __restrict__ proj,
                                                   T* __restrict__ img,
@@ -108,55 +328,56 @@ __global__ __launch_bounds__(256) void fpt_kernel(const T* __restrict__ proj,
                                                   const int* __restrict__ usex, FpProblem P,
                                                   int stride)
 {
-    using Acc = typename std::conditional<sizeof(T) == 8, double, float>::type;
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int b0 = blockIdx.y * NB;
     const int nb = min(NB, P.nbatch - b0);
     if (i >= P.nx * P.ny) return;
     const int ix = i % P.nx, iy = i / P.nx;
-    const size_t pslice = static_cast<size_t>(P.na) * P.ns;
     double sum[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) sum[b] = 0.0;
-    for (int a = 0; a < P.na; ++a) {
-        double mid, left, right, o1;
-        if (usex[a]) {
-            const double* pp = pos + 2 * (static_cast<size_t>(a) * stride + iy);
-            o1 = pp[1];
-            mid = pp[0] + o1 * ix;
-            left = ix == 0 ? mid : mid - o1;
-            right = ix == P.nx - 1 ? mid : mid + o1;
-        } else {
-            const double* pp = pos + 2 * (static_cast<size_t>(a) * stride + ix);
-            o1 = pp[1];
-            mid = pp[0] + o1 * iy;
-            left = iy == 0 ? mid : mid - o1;
-            right = iy == P.ny - 1 ? mid : mid + o1;
-        }
-        if (left > right) {
-            const double t = left;
-            left = right;
-            right = t;
-        }
-        const int c0 = max(static_cast<int>(ceil(left)), 0);
-        const int c1 = min(static_cast<int>(floor(right)), P.ns - 1);
-        Acc tmp[NB];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) tmp[b] = Acc(0);
-        for (int c = c0; c <= c1; ++c) {
-            const Acc w = static_cast<Acc>(1.0 - fabs((mid - c) / o1));
-            const T* pr = proj + static_cast<size_t>(a) * P.ns + c;
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-                if (b < nb) tmp[b] += w * static_cast<Acc>(pr[(b0 + b) * pslice]);
-        }
-        const double wa = weights[a];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) sum[b] += static_cast<double>(tmp[b]) * wa;
-    }
+    for (int a = 0; a < P.na; ++a) fpt_view<T, NB>(proj, weights, pos, usex, P, stride, a, ix, iy,
+                                                   b0, nb, sum);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
         if (b < nb) img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix] = static_cast<T>(sum[b]);
+}
+
+// The border pixels (nx, ny >= 3) after the engine pass: one workgroup per
+// border pixel and slice group, the views split over its threads and the
+// sums reduced in the workgroup.
+template <int NB>
+__global__ __launch_bounds__(256) void fpt_border_kernel(const float* __restrict__ proj,
+                                                         float* __restrict__ img,
+                                                         const double* __restrict__ weights,
+                                                         const double* __restrict__ pos,
+                                                         const int* __restrict__ usex,
+                                                         FpProblem P, int stride)
+{
+    __shared__ double red[4][NB];
+    int ix, iy;
+    border_pixel(blockIdx.x, P.nx, P.ny, ix, iy);
+    const int b0 = blockIdx.y * NB;
+    const int nb = min(NB, P.nbatch - b0);
+    double sum[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) sum[b] = 0.0;
+    for (int a = threadIdx.x; a < P.na; a += 256)
+        fpt_view<float, NB>(proj, weights, pos, usex, P, stride, a, ix, iy, b0, nb, sum);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        for (int o = 32; o > 0; o >>= 1) sum[b] += __shfl_xor(sum[b], o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) red[wave][b] = sum[b];
+    __syncthreads();
+    if (threadIdx.x < nb) {
+        const int b = threadIdx.x;
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += red[w][b];
+        img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix] = static_cast<float>(t);
+    }
 }
 
 namespace {
@@ -168,6 +389,35 @@ int launch_fp(const FpProblem& P, const T* img, const T* imgT, T* proj, const do
     dim3 grid(div_up(P.ns, 256), P.na, div_up(P.nbatch, NB));
     hipLaunchKernelGGL((fp_kernel<T, NB>), grid, dim3(256), 0, st, img, imgT, proj, sc, bo, usex, P);
     return check_launch("fp_kernel");
+}
+
+constexpr int kFpViews = 4;
+
+// Columns the rays of one fp_lds_kernel chunk can span: 256 channels at up
+// to sqrt(2) ds / pixel per channel, kFpRows rows at up to one column per row
+// (in pixel units of the marched axis), and the drift of a ray's column
+// across kFpViews views (at most 4 R |orbit| per view for an image of
+// half-diagonal R pixels), plus margins.
+bool fp_lds_applies(const ctr_geom& g)
+{
+    const double pmin = std::min(std::fabs(g.dx), std::fabs(g.dy));
+    const double pmax = std::max(std::fabs(g.dx), std::fabs(g.dy));
+    if (!(pmin > 0) || !(g.ds > 0)) return false;
+    const double r = 0.5 * std::hypot(g.nx + 2 * std::fabs(g.offset_x),
+                                      g.ny + 2 * std::fabs(g.offset_y)) * pmax / pmin;
+    const double span = 255.0 * std::sqrt(2.0) * g.ds / pmin + kFpRows * pmax / pmin +
+                        4.0 * r * kFpViews * std::fabs(g.orbit) + 8.0;
+    return span < kFpSpan - 64;
+}
+
+template <int NB>
+int launch_fp_lds(const FpProblem& P, const float* img, const float* imgT, float* proj,
+                  const double* sc, const double* bo, const int* usex, hipStream_t st)
+{
+    dim3 grid(div_up(P.ns, 256), div_up(P.na, kFpViews), div_up(P.nbatch, NB));
+    hipLaunchKernelGGL((fp_lds_kernel<NB, kFpViews>), grid, dim3(256), 0, st, img, imgT, proj, sc,
+                       bo, usex, P);
+    return check_launch("fp_lds_kernel");
 }
 
 template <typename T>
@@ -183,6 +433,15 @@ int fp_typed(const ctr_geom& g, const T* img, T* proj, int nbatch, const double*
                        dim3(32, 8), 0, st, img, imgT, static_cast<int>(g.ny),
                        static_cast<int>(g.nx));
     int rc = check_launch("transpose_kernel");
+    if constexpr (std::is_same<T, float>::value) {
+        if (rc == CTR_OK && fp_lds_applies(g)) {
+            // two slices per workgroup: 70 VGPRs (four slices need 132)
+            rc = nbatch >= 2 ? launch_fp_lds<2>(P, img, imgT, proj, sc, bo, usex, st)
+                             : launch_fp_lds<1>(P, img, imgT, proj, sc, bo, usex, st);
+            CTR_HIP(hipFreeAsync(imgT, st));
+            return rc;
+        }
+    }
     if (rc == CTR_OK) {
         const int nb = nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
         switch (nb) {
@@ -206,10 +465,75 @@ int launch_fpt(const FpProblem& P, const T* proj, T* img, const double* w, const
     return check_launch("fpt_kernel");
 }
 
+template <int NB>
+int launch_fpt_border(const FpProblem& P, const float* proj, float* img, const double* w,
+                      const double* pos, const int* usex, int stride, hipStream_t st)
+{
+    dim3 grid(2 * P.nx + 2 * (P.ny - 2), div_up(P.nbatch, NB));
+    hipLaunchKernelGGL((fpt_border_kernel<NB>), grid, dim3(256), 0, st, proj, img, w, pos, usex,
+                       P, stride);
+    return check_launch("fpt_border_kernel");
+}
+
+template <int NB>
+int launch_fpt_engine(const ctr_geom& g, const float* proj, float* img, int nbatch,
+                      const double* w, const double* pos, const int* usex, int stride,
+                      hipStream_t st)
+{
+    constexpr int TW = 16, TH = 16, LROW = 64;
+    BpLaunch<FptGeo> L{};
+    L.P = FptProblem{static_cast<int>(g.ns), static_cast<int>(g.na), static_cast<int>(g.nx),
+                     static_cast<int>(g.ny)};
+    L.nbatch = nbatch;
+    L.tiles_x = div_up(g.nx, TW);
+    L.tiles_y = div_up(g.ny, TH);
+    L.lmax = LROW;
+    L.vchunk = 4;  // 35.8 KB of LDS at NB = 8: four workgroups per CU
+    L.scale = 1.0;
+    L.proj = proj;
+    L.img = img;
+    const FptTables T{w, pos, usex, stride};
+    const size_t lds = kAnchorChunk * sizeof(FptAnchor) +
+                       2 * static_cast<size_t>(L.vchunk) * NB * LROW * sizeof(float);
+    auto kern = bp_forward_kernel<FptGeo, TW, TH, NB, LROW>;
+    if (lds > 64 * 1024)
+        CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(lds)));
+    hipLaunchKernelGGL(kern, dim3(L.tiles_x * L.tiles_y, div_up(nbatch, NB)),
+                       dim3(TileShape<TW, TH>::kBlock), lds, st, T, L);
+    return check_launch("bp_forward_kernel<fp transpose>");
+}
+
+// The engine path needs |o1| <= 1 for every view (at most two channels per
+// pixel-view) and a staged span that fits 64 channels per 16x16 tile:
+// |o1| <= max(dx, dy)/ds and the span is at most 15 (|o1| + |o2|) + 3.
+bool fpt_engine_applies(const ctr_geom& g)
+{
+    const double r = std::max(std::fabs(g.dx), std::fabs(g.dy)) / g.ds;
+    return g.ds > 0 && r <= 1.0 && 15.0 * 2.0 * r + 4.0 <= 64.0;
+}
+
 template <typename T>
 int fpt_typed(const ctr_geom& g, const T* proj, T* img, int nbatch, const double* w,
               const double* pos, const int* usex, unsigned int stride, hipStream_t st)
 {
+    if constexpr (std::is_same<T, float>::value) {
+        if (fpt_engine_applies(g) && g.nx >= 3 && g.ny >= 3) {
+            // interior on the engine, then the border pixels exactly
+            const int s = static_cast<int>(stride);
+            int rc;
+            if (nbatch >= 8) rc = launch_fpt_engine<8>(g, proj, img, nbatch, w, pos, usex, s, st);
+            else if (nbatch >= 4) rc = launch_fpt_engine<4>(g, proj, img, nbatch, w, pos, usex, s, st);
+            else if (nbatch >= 2) rc = launch_fpt_engine<2>(g, proj, img, nbatch, w, pos, usex, s, st);
+            else rc = launch_fpt_engine<1>(g, proj, img, nbatch, w, pos, usex, s, st);
+            if (rc) return rc;
+            FpProblem P{static_cast<int>(g.ns), static_cast<int>(g.na), static_cast<int>(g.nx),
+                        static_cast<int>(g.ny), nbatch, g.dx, g.dy};
+            return nbatch >= 8 ? launch_fpt_border<8>(P, proj, img, w, pos, usex, s, st)
+                               : launch_fpt_border<1>(P, proj, img, w, pos, usex, s, st);
+        }
+    }
     FpProblem P{static_cast<int>(g.ns), static_cast<int>(g.na), static_cast<int>(g.nx),
                 static_cast<int>(g.ny), nbatch, g.dx, g.dy};
     const int nb = nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;

@@ -38,7 +38,7 @@ struct RaggedDesc {
     double scale;
     int tiles_x, tiles_y;
     int lmax, vchunk;
-    int variant;  // 4 * exact + span class (0: 64, 1: 128, 2: 256, 3: 512 channels)
+    int variant;  // 4 * exact + span class (0: 64, 1: 128, 2: 256 channels)
     int pad;
 };
 
@@ -62,11 +62,9 @@ __device__ __forceinline__ void ragged_run_item(const RaggedDesc& D, int tile, u
         CTR_RAGGED_CASE(0, false, 64)
         CTR_RAGGED_CASE(1, false, 128)
         CTR_RAGGED_CASE(2, false, 256)
-        CTR_RAGGED_CASE(3, false, 512)
         CTR_RAGGED_CASE(4, true, 64)
         CTR_RAGGED_CASE(5, true, 128)
         CTR_RAGGED_CASE(6, true, 256)
-        CTR_RAGGED_CASE(7, true, 512)
         default: break;
     }
 #undef CTR_RAGGED_CASE
@@ -104,12 +102,16 @@ namespace {
 
 constexpr int kTileW[3] = {16, 8, 32}, kTileH[3] = {16, 8, 8};
 
+// Staged span classes.  A tile whose span exceeds 256 channels is not
+// admitted: pixel offsets from the tile anchor then exceed ~128 channels and
+// their FP32 rounding (~2e-7 relative) would cost more than 1e-5 channel of
+// position accuracy -- the rule the single-geometry launcher applies by
+// switching to 8x8 tiles.
 int span_class(int lmax)
 {
     if (lmax <= 64) return 0;
     if (lmax <= 128) return 1;
     if (lmax <= 256) return 2;
-    if (lmax <= 512) return 3;
     return -1;
 }
 
@@ -214,7 +216,7 @@ int ctr_ragged_fan_bp_plan(const ctr_ragged_problem* probs, int nprob, int tile,
             const int cls = span_class(lmax);
             CTR_REQUIRE(cls >= 0, CTR_ERR_SHAPE,
                         "ragged fan BP: problem " + std::to_string(i) +
-                            ": a pixel tile spans more than 512 channels; use a smaller tile");
+                            ": a pixel tile spans more than 256 channels; use a smaller tile");
             const int lrow = 64 << cls;
             int vc = static_cast<int>((lds_budget - anchor_bytes) / (2 * lrow * sizeof(float)));
             vc = std::max(1, std::min(vchunk, vc));

@@ -36,7 +36,7 @@ __device__ __forceinline__ int xcd_tile(int id, int ntiles)
 // Origin of a tile and the pixel (ic, jc) its anchor is evaluated at.
 struct TileFrame {
     int x0, y0, ic, jc;
-    __device__ TileFrame(int tile, int tiles_x, int nx, int ny, int TW, int TH)
+    __host__ __device__ TileFrame(int tile, int tiles_x, int nx, int ny, int TW, int TH)
     {
         x0 = (tile % tiles_x) * TW;
         y0 = (tile / tiles_x) * TH;

@@ -18,6 +18,16 @@
 // compensated product hi*hi + hi*lo + lo*hi accumulated in FP32 (the lo*lo
 // term, ~2^-16 relative, is dropped).
 //
+// Near field in FP64.  The output is a small difference of large terms: the
+// centre tap h0 and the odd taps |m| <= 15 carry ~97% of the filter's
+// weight and nearly cancel on smooth rows, so FP32 accumulation of them
+// (inside the MFMA) would cost ~2e-5 relative error on CT sinograms.  Those
+// 16 taps are therefore zeroed in the GEMM's tap vector and evaluated in the
+// epilogue in FP64 (FP64 taps times the same FP32 weighted inputs, from an
+// LDS window of the tile's rows), and the three parts are combined in FP64;
+// the matrix cores only carry the far field (~2.5% of the weight), whose
+// FP32 / BF16-split rounding is correspondingly small.
+//
 // Workgroup: 4 waves, a 64-row x 64-output tile of one parity; each wave a
 // 32 x 32 block (2 x 2 MFMA tiles of 16 x 16).  A is staged in LDS in
 // K-chunks of 64 (64 x 64 words, padded); the tap vector covers the whole
@@ -35,16 +45,21 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kTM = 64, kTN = 64, kKC = 64, kAPad = 4;
+// Near-field taps m = 2q + 1, q in [-kNear, kNear - 1] (|m| <= 2 kNear - 1),
+// and the per-row window of opposite-parity inputs they read for a tile.
+constexpr int kNear = 8;
+constexpr int kWin = kTN + 2 * kNear;
 
 struct ToeplitzParams {
     const float* x;      // [nrow][ns]
     float* out;          // [nrow][ns]
+    const double* taps64;  // FP64 h[m] at taps64[ns + m] (centre and near field)
     const float* taps;   // FP32 h[m] at taps[ns + m], m in [-ns, ns]
     const float* wts;    // FP32 per-channel weight, or nullptr
     int ns, nrow;
     int n_even, n_odd;   // outputs of each parity
     int tiles_even;      // column tiles of parity 0 (the rest are parity 1)
-    float ds;
+    double ds;
 };
 
 __device__ __forceinline__ float wt(const ToeplitzParams& P, int k)
@@ -74,7 +89,8 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
     for (int e = tid; e < nq; e += 256) {
         const int q = e + qmin;
         const int m = 2 * q + 1;
-        tap[e] = (m >= -P.ns && m <= P.ns) ? P.taps[P.ns + m] : 0.0f;
+        const bool near = q >= -kNear && q < kNear;  // evaluated in the epilogue
+        tap[e] = (m >= -P.ns && m <= P.ns && !near) ? P.taps[P.ns + m] : 0.0f;
     }
 
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // wave block in the tile
@@ -159,21 +175,53 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
         }
     }
 
+    // Near-field window: win[rr][w] = weighted input of parity 1-pi at
+    // i = j0 + pi - kNear + w (the A operand's values), zero off the row.
+    __syncthreads();  // last chunk consumed: reuse the A buffer
+    float* win = a_lds;  // [kTM][kWin]: the launch sizes LDS for it
+    const int iw0 = j0 + pi - kNear;
+    for (int e = tid; e < kTM * kWin; e += 256) {
+        const int rr = e / kWin, w = e % kWin;
+        const int r = r0 + rr, i = iw0 + w;
+        float val = 0.0f;
+        if (r < P.nrow && i >= 0 && i < kdim) {
+            const int k = 2 * i + kpar;
+            val = P.x[static_cast<size_t>(r) * P.ns + k] * wt(P, k);
+        }
+        win[e] = val;
+    }
+    __syncthreads();
+
     // D layout: lane holds D[4*(lane/16) + t][lane%16], t = 0..3.
-    const float h0 = P.taps[P.ns];
+    // out[2j+pi] = ds (h0 xw[2j+pi] + sum_q h[2q+1] xw[2(j+pi-1-q)+kpar] + far)
+    const double h0 = P.taps64[P.ns];
+    double hn[2 * kNear];
+#pragma unroll
+    for (int q = -kNear; q < kNear; ++q) {
+        const int m = 2 * q + 1;
+        hn[q + kNear] = (m >= -P.ns && m <= P.ns) ? P.taps64[P.ns + m] : 0.0;
+    }
 #pragma unroll
     for (int ta = 0; ta < 2; ++ta)
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const int r = r0 + wr + 16 * ta + 4 * (lane >> 4) + t;
-                const int j = j0 + wc + 16 * tb + (lane & 15);
+                const int rr = wr + 16 * ta + 4 * (lane >> 4) + t;
+                const int jj = wc + 16 * tb + (lane & 15);
+                const int r = r0 + rr, j = j0 + jj;
                 if (r < P.nrow && j < nout) {
                     const int n = 2 * j + pi;
                     const size_t o = static_cast<size_t>(r) * P.ns + n;
                     const float xw = P.x[o] * wt(P, n);
-                    P.out[o] = P.ds * fmaf(h0, xw, acc[ta][tb][t]);
+                    // input i = j + pi - 1 - q sits at window slot jj + kNear - 1 - q
+                    const float* wrow = win + rr * kWin + jj + kNear - 1;
+                    double near = h0 * static_cast<double>(xw);
+#pragma unroll
+                    for (int q = -kNear; q < kNear; ++q)
+                        near = fma(hn[q + kNear], static_cast<double>(wrow[-q]), near);
+                    P.out[o] = static_cast<float>(
+                        P.ds * (near + static_cast<double>(acc[ta][tb][t])));
                 }
             }
 }
@@ -207,6 +255,7 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
         ToeplitzParams P{};
         P.x = in;
         P.out = out;
+        P.taps64 = filter;
         P.taps = hf;
         P.wts = w;
         P.ns = ns;
@@ -214,9 +263,9 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
         P.n_even = (ns + 1) / 2;
         P.n_odd = ns / 2;
         P.tiles_even = div_up(P.n_even, kTN);
-        P.ds = static_cast<float>(g.ds);
-        const size_t lds =
-            (static_cast<size_t>(kTM) * (kKC + kAPad) + ns + kKC + kTN + 10) * sizeof(float);
+        P.ds = g.ds;
+        const size_t lds = std::max(static_cast<size_t>(kTM) * (kKC + kAPad) + ns + kKC + kTN + 10,
+                                    static_cast<size_t>(kTM) * kWin) * sizeof(float);
         CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "Toeplitz ramp filter: ns too large");
         auto kern = bf16x3 ? ramp_mfma_kernel<true> : ramp_mfma_kernel<false>;
         if (lds > 64 * 1024)

@@ -3,7 +3,7 @@
 At the full workload (8 x 512^2, 720 views over 180 deg, 729 channels):
 dot-product adjoint gap <= 1e-6 on seeded Gaussian x, y; central finite
 differences (eps = 1e-3 max|x|) of 0.5 ||FP x - y||^2 against the autograd
-gradient on 16 coordinates (relative 1e-3 in float32, 1e-7 in float64);
+gradient on 16 coordinates (relative 1e-2 in float32, 1e-7 in float64);
 the loss decreases monotonically over gradient steps.  On a small scan the
 GPU iteration is compared with the same iteration run on the oracle's FP
 and transpose in float64.
@@ -43,7 +43,11 @@ def test_adjoint_full_size(env):
     assert gap < 1e-6, gap
 
 
-@pytest.mark.parametrize("dtype,tol", [("float32", 1e-3), ("float64", 1e-7)])
+# float32: every perturbed ray sum (~512 samples) is re-rounded after the
+# perturbed sample, and that rounding does not cancel between the two
+# evaluations; at eps = 1e-3 max|x| it is ~1e-3 of the difference, so the
+# float32 check is held to 1e-2 and the float64 one (1e-7) is the tight one.
+@pytest.mark.parametrize("dtype,tol", [("float32", 1e-2), ("float64", 1e-7)])
 def test_finite_differences_full_size(env, dtype, tol):
     torch, recon = env
     dt = getattr(torch, dtype)
@@ -106,3 +110,20 @@ def test_gradient_descent_matches_oracle_iteration(env):
                                   torch.from_numpy(y[None].astype(np.float32)).cuda(), scan,
                                   steps=15, alpha=alpha)
     assert rel_l2(x[0].cpu().numpy(), xo) < 1e-5
+
+
+def test_fp_and_transpose_full_size_vs_oracle(env):
+    torch, recon = env
+    scan = recon.ParallelScan()
+    fp, fpt = recon.projector(scan)
+    g = O.geom(ns=scan.ns, na=scan.na, ds=scan.ds, orbit=math.pi / scan.na, nx=scan.n,
+               ny=scan.n, dx=scan.dx, dy=scan.dx)
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((scan.n, scan.n)).astype(np.float32)
+    y = rng.standard_normal((scan.na, scan.ns)).astype(np.float32)
+    got_fp = fp(torch.from_numpy(x)[None].cuda())[0].cpu().numpy()
+    got_fpt = fpt(torch.from_numpy(y)[None].cuda())[0].cpu().numpy()
+    assert rel_l2(got_fp, O.fp(g, x, O.fp_prep(g))) < 1e-5
+    # float64 oracle transpose: the reference's float one accumulates in T
+    want = O.fp_grad(g, y.astype(np.float64), O.fp_grad_prep(g))
+    assert rel_l2(got_fpt, want) < 1e-5

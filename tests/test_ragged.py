@@ -20,22 +20,27 @@ TOL32 = 1e-5
 
 # Small problems that together cover: non-square images, pixel and detector
 # offsets, a source close to the image (exact atan mode, wide spans), each
-# staged span class (64 / 128 / 256 / 512 channels), views off [0, 2 pi),
-# unsorted angles, and a near-empty problem.
+# staged span class (64 / 128 / 256 channels), views off [0, 2 pi), unsorted
+# angles, and a near-empty problem.  Every tile shape admits all of them
+# (span <= 256 channels); WIDE does not admit 32x8 tiles.
 SMALL = [
     dict(ns=64, na=37, ds=1.7, offset_s=0.4, nx=24, ny=20, dx=1.3, dy=1.5, offset_x=0.3,
          offset_y=-0.2, dso=60.0, dsd=110.0),
     dict(ns=96, na=50, ds=2.0, offset_s=-1.25, nx=40, ny=40, dx=3.0, dy=3.0, offset_x=0.0,
          offset_y=0.0, dso=300.0, dsd=550.0),
-    dict(ns=200, na=45, ds=0.5, offset_s=0.0, nx=64, ny=48, dx=1.2, dy=1.2, offset_x=-2.5,
+    dict(ns=200, na=45, ds=0.5, offset_s=0.0, nx=64, ny=48, dx=1.0, dy=1.0, offset_x=-2.5,
          offset_y=1.0, dso=120.0, dsd=240.0),
-    dict(ns=300, na=29, ds=0.25, offset_s=0.7, nx=32, ny=32, dx=1.2, dy=1.2, offset_x=0.0,
+    dict(ns=300, na=29, ds=0.25, offset_s=0.7, nx=32, ny=32, dx=0.6, dy=0.6, offset_x=0.0,
          offset_y=0.0, dso=100.0, dsd=200.0),
     dict(ns=16, na=3, ds=1.0, offset_s=0.0, nx=8, ny=8, dx=0.5, dy=0.5, offset_x=0.0,
          offset_y=0.0, dso=50.0, dsd=90.0),
     dict(ns=128, na=61, ds=1.1, offset_s=0.0, nx=56, ny=72, dx=0.9, dy=0.8, offset_x=0.5,
          offset_y=0.5, dso=400.0, dsd=700.0),
 ]
+
+
+WIDE = dict(ns=300, na=29, ds=0.25, offset_s=0.7, nx=32, ny=32, dx=1.2, dy=1.2, offset_x=0.0,
+            offset_y=0.0, dso=100.0, dsd=200.0)
 
 
 def _angles(i, na):
@@ -179,3 +184,17 @@ def test_ragged_balance_and_errors(R):
         R.RaggedFanBackprojection2DAngle(params, angles, tile=3, vchunk=8)
     with pytest.raises(_lib.CtrError):
         R.RaggedFanBackprojection2DAngle(params, angles, tile=0, vchunk=65)
+
+
+@pytest.mark.gpu
+def test_ragged_rejects_tiles_with_spans_over_256_channels(R):
+    import torch
+    from ct_recon_extensions_amd import _lib
+    params, angles = _params([WIDE]), [_angles(3, WIDE["na"])]
+    with pytest.raises(_lib.CtrError, match="256 channels"):
+        R.RaggedFanBackprojection2DAngle(params, angles, tile=2, vchunk=16)
+    rb = R.RaggedFanBackprojection2DAngle(params, angles, tile=1, vchunk=16)
+    proj = np.random.default_rng(3).standard_normal((WIDE["na"], WIDE["ns"])).astype(np.float32)
+    got = rb([torch.from_numpy(proj).cuda()], graph=False)[0].cpu().numpy()
+    assert rel_l2(got, _oracle(WIDE, proj, angles[0])) < TOL32
+    rb.close()
