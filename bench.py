@@ -132,16 +132,25 @@ def profiled_traffic():
     return None
 
 
-def profile_bp(torch, F):
-    """Fan BP only (for rocprofv3 --pmc): three launches on random sinograms of
-    the workload's shape, then a copy of a 256 MiB buffer that was evicted
-    from L2 and MALL by writing 1 GiB elsewhere: its FETCH_SIZE / WRITE_SIZE
-    against 256 MiB calibrate the counters."""
+def profile_kernels(torch, F, which):
+    """One kernel family alone (for rocprofv3 --pmc): three launches on random
+    inputs of the workload's shape, then a copy of a 256 MiB buffer that was
+    evicted from L2 and MALL by writing 1 GiB elsewhere: its FETCH_SIZE /
+    WRITE_SIZE against 256 MiB calibrate the counters."""
     cfg = CFG
     g = torch.Generator(device="cuda").manual_seed(0)
-    proj = torch.randn((cfg["nbatch"], cfg["na"], cfg["ns"]), generator=g, device="cuda")
-    for _ in range(3):
-        img = F.fan_backprojection_2d(proj, **kw_fan(cfg))
+    if which == "lsq":
+        from ct_recon_extensions_amd import recon
+        scan = recon.ParallelScan()
+        fp, fpt = recon.projector(scan)
+        x = torch.randn((8, scan.n, scan.n), generator=g, device="cuda")
+        for _ in range(3):
+            img = fpt(fp(x))
+    else:
+        proj = torch.randn((cfg["nbatch"], cfg["na"], cfg["ns"]), generator=g, device="cuda")
+        for _ in range(3):
+            img = (F.fan_backprojection_2d(proj, **kw_fan(cfg)) if which == "bp"
+                   else F.fan_filter(proj, **kw_filter(cfg), impl="mfma"))
     torch.cuda.synchronize()
     src = torch.randn(64 * 1024 * 1024, generator=g, device="cuda")
     evict = torch.zeros(256 * 1024 * 1024, device="cuda")
@@ -149,7 +158,7 @@ def profile_bp(torch, F):
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"profile": "fan_bp", "checksum": float(img.double().abs().sum()),
+    print(json.dumps({"profile": which, "checksum": float(img.double().abs().sum()),
                       "copy_bytes": src.numel() * 4, "evict_bytes": evict.numel() * 4}))
 
 
@@ -275,6 +284,11 @@ def main():
     ap.add_argument("--profile-bp", action="store_true",
                     help="counter-collection mode: three fan BP launches of the workload, then a "
                          "256 MiB device copy from a cold source (calibrates FETCH/WRITE_SIZE)")
+    ap.add_argument("--profile", choices=("bp", "filter", "lsq"), default=None,
+                    help="counter-collection mode for one kernel family: 'bp' as --profile-bp; "
+                         "'filter' three fused fan-weighted ramp filters (FP32 MFMA) of the "
+                         "workload; 'lsq' three FP and FP^T launches of the least-squares "
+                         "workload; each followed by the calibration copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path shards by slice with no exchange, so "
@@ -287,8 +301,8 @@ def main():
     from ct_recon_extensions_amd import _lib
     from ct_recon_extensions_amd.geometry import FanBackprojection2DParam, device_tables
 
-    if args.profile_bp:
-        profile_bp(torch, F)
+    if args.profile_bp or args.profile is not None:
+        profile_kernels(torch, F, "bp" if args.profile_bp else args.profile)
         return
 
     lib = _lib.load()

@@ -111,17 +111,18 @@ __global__ __launch_bounds__(256) void fp_kernel(const T* __restrict__ img,
                 static_cast<T>(static_cast<double>(sum[b]) * dw);
 }
 
-// FP with LDS-staged image rows (float32).  A workgroup owns 256 consecutive
-// channels (one per thread) of V consecutive views and NB slices, and marches
-// the rows of the image (x-branch views) or of its transpose (y-branch
-// views) in chunks of kFpRows: per chunk it stages, for all NB slices, the
-// span of columns its V x 256 rays can touch (DMA from global straight into
-// LDS; zero outside the row), then every thread advances its V rays through
-// those rows.  A staged pixel is read by ~V rays instead of every view
-// re-reading the image through L1/L2.  Each ray keeps the reference's FP64
-// position accumulation (x += offset per row), inclusion test, taps and FP32
-// sum order, so the result is bit-identical to fp_kernel.  Views of the other
-// branch in the group are handled in a second pass.
+// FP with LDS-staged image rows (float32).  A workgroup owns up to 256
+// consecutive channels (one per thread) of V consecutive views and NB
+// slices, and marches the rows of the image (x-branch views) or of its
+// transpose (y-branch views) in chunks of kFpRows: per chunk it stages, for
+// all NB slices, the span of columns its rays can touch (DMA from global
+// straight into LDS, zero outside the row; slice rows kFpSpan words apart),
+// then every thread advances its V rays through those rows.  A staged pixel
+// is read by ~V rays instead of every view re-reading the image through
+// L1/L2, and a ray's two taps for a pair of slices come from one
+// ds_read2st64_b32.  Each ray keeps the reference's FP64 position
+// accumulation (x += offset per row) and inclusion test; the lerp and the
+// sum are FP32.  Views of the other branch in the group take a second pass.
 constexpr int kFpRows = 4;
 constexpr int kFpSpan = 512;  // staged columns per row (host checks the bound)
 
@@ -131,12 +132,17 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
                                                      float* __restrict__ proj,
                                                      const double* __restrict__ sincos,
                                                      const double* __restrict__ bo,
-                                                     const int* __restrict__ usex, FpProblem P)
+                                                     const int* __restrict__ usex, FpProblem P,
+                                                     int cpb)
 {
+    static_assert(NB % 2 == 0 || NB == 1, "slices are read in pairs");
     __shared__ __align__(16) float rows[kFpRows * NB * kFpSpan];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int s0 = blockIdx.x * 256, s = s0 + tid;
-    const int slast = min(s0 + 255, P.ns - 1);
+    // the wave index is uniform: keeps the DMA's buffer descriptor in SGPRs
+    // (no per-lane waterfall around buffer_load ... lds)
+    const int tid = threadIdx.x, lane = tid & 63, wave = uniform_i(tid >> 6);
+    const int s0 = blockIdx.x * cpb, s = s0 + tid;
+    const int slast = min(s0 + cpb, P.ns) - 1;
+    const bool has_ray = tid < cpb && s < P.ns;
     const int a0 = blockIdx.y * V;
     const int b0 = blockIdx.z * NB;
     const int nb = min(NB, P.nbatch - b0);
@@ -144,7 +150,6 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
 
     for (int pass = 0; pass < 2; ++pass) {
         const bool xb = pass == 0;
-        // views of this pass
         bool act[V];
         int nact = 0;
 #pragma unroll
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
         for (int v = 0; v < V; ++v) {
             const int a = min(a0 + v, P.na - 1);
             const double* r = bo + 2 * static_cast<size_t>(a) * P.ns;
-            x[v] = s < P.ns ? r[2 * s] : 0.0;
+            x[v] = has_ray ? r[2 * s] : 0.0;
             off[v] = r[1];
             blo[v] = r[2 * s0];
             bhi[v] = r[2 * slast];
@@ -192,8 +197,10 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
             }
             const int cmin = max(0, static_cast<int>(floor(xmin)) - 1);
             const int cmax = min(rowlen - 1, static_cast<int>(floor(xmax)) + 2);
-            // 64-column DMA rows; the host keeps cmax - cmin < kFpSpan
-            const int nload = min((cmax - cmin + 64) / 64, kFpSpan / 64);
+            // 64-column DMA rows covering columns cmin .. cmax + 1 (the right tap
+            // of a sample at x == nx-1 reads cmax + 1: zero past the row); the
+            // host keeps cmax - cmin + 2 <= kFpSpan
+            const int nload = min((cmax - cmin + 65) / 64, kFpSpan / 64);
             // stage rows (k0 + kk, slice b) -> rows[(kk * NB + b) * kFpSpan + c]
             for (int e = wave; e < nr * NB; e += 4) {
                 const int kk = e / NB, b = e - kk * NB;
@@ -210,23 +217,27 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
             }
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            if (s < P.ns) {
+            if (has_ray) {
 #pragma unroll 1
                 for (int kk = 0; kk < nr; ++kk) {
                     const float* rk = rows + kk * NB * kFpSpan - cmin;
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
                         if (act[v] && x[v] >= 0 && x[v] <= hi) {
-                            const double fl = floor(x[v]);
-                            const int i1 = static_cast<int>(fl);
-                            const int i2 = min(i1 + 1, rowlen - 1);
-                            const float u = static_cast<float>(x[v] - fl);
+                            // x >= 0: truncation is the floor
+                            const int i1 = static_cast<int>(x[v]);
+                            const float u = static_cast<float>(x[v] - static_cast<double>(i1));
+                            const float w0 = 1.0f - u;
+                            // right tap through a base the compiler cannot fold, so the
+                            // slice pairs load as ds_read2st64_b32 (rows kFpSpan apart);
+                            // at x == hi its weight u is 0 and it reads a staged zero
+                            int one = 1;
+                            asm volatile("" : "+v"(one));
+                            const float* q0 = rk + i1;
+                            const float* q1 = rk + i1 + one;
 #pragma unroll
-                            for (int b = 0; b < NB; ++b) {
-                                const float p0 = rk[b * kFpSpan + i1];
-                                const float p1 = rk[b * kFpSpan + i2];
-                                sum[v][b] += p0 + u * (p1 - p0);
-                            }
+                            for (int b = 0; b < NB; ++b)
+                                sum[v][b] = fmaf(u, q1[b * kFpSpan], fmaf(w0, q0[b * kFpSpan], sum[v][b]));
                         }
                         x[v] += off[v];
                     }
@@ -237,7 +248,7 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
                     for (int k = 0; k < nr; ++k) x[v] += off[v];
             }
         }
-        if (s < P.ns) {
+        if (has_ray) {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 if (!act[v]) continue;
@@ -391,7 +402,8 @@ int launch_fp(const FpProblem& P, const T* img, const T* imgT, T* proj, const do
     return check_launch("fp_kernel");
 }
 
-constexpr int kFpViews = 4;
+constexpr int kFpViews = 2;
+constexpr int kFpSlices = 4;
 
 // Columns the rays of one fp_lds_kernel chunk can span: 256 channels at up
 // to sqrt(2) ds / pixel per channel, kFpRows rows at up to one column per row
@@ -414,9 +426,12 @@ template <int NB>
 int launch_fp_lds(const FpProblem& P, const float* img, const float* imgT, float* proj,
                   const double* sc, const double* bo, const int* usex, hipStream_t st)
 {
-    dim3 grid(div_up(P.ns, 256), div_up(P.na, kFpViews), div_up(P.nbatch, NB));
+    // channels split evenly over ceil(ns / 256) workgroups
+    const int nblk = div_up(P.ns, 256);
+    const int cpb = div_up(P.ns, nblk);
+    dim3 grid(nblk, div_up(P.na, kFpViews), div_up(P.nbatch, NB));
     hipLaunchKernelGGL((fp_lds_kernel<NB, kFpViews>), grid, dim3(256), 0, st, img, imgT, proj, sc,
-                       bo, usex, P);
+                       bo, usex, P, cpb);
     return check_launch("fp_lds_kernel");
 }
 
@@ -435,9 +450,9 @@ int fp_typed(const ctr_geom& g, const T* img, T* proj, int nbatch, const double*
     int rc = check_launch("transpose_kernel");
     if constexpr (std::is_same<T, float>::value) {
         if (rc == CTR_OK && fp_lds_applies(g)) {
-            // two slices per workgroup: 70 VGPRs (four slices need 132)
-            rc = nbatch >= 2 ? launch_fp_lds<2>(P, img, imgT, proj, sc, bo, usex, st)
-                             : launch_fp_lds<1>(P, img, imgT, proj, sc, bo, usex, st);
+            rc = nbatch >= kFpSlices ? launch_fp_lds<kFpSlices>(P, img, imgT, proj, sc, bo, usex, st)
+                 : nbatch >= 2       ? launch_fp_lds<2>(P, img, imgT, proj, sc, bo, usex, st)
+                                     : launch_fp_lds<1>(P, img, imgT, proj, sc, bo, usex, st);
             CTR_HIP(hipFreeAsync(imgT, st));
             return rc;
         }

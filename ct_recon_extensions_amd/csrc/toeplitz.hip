@@ -73,10 +73,13 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
     // taps ho[q] at tap[q - qmin] for every q a tile can touch (zero where
     // |2q+1| > ns): q = j - i + pi - 1 with j < nout + kTN, i < kdim + kKC
     extern __shared__ __align__(16) unsigned char smem[];
-    float* a_lds = reinterpret_cast<float*>(smem);  // [kTM][kKC + kAPad]
-    float* tap = a_lds + kTM * (kKC + kAPad);
     const int qmin = -(P.ns / 2) - kKC - 4;
     const int nq = P.ns + kKC + kTN + 10;
+    // LDS: channel weights [ns] | taps [nq] | A chunk [kTM][kKC + kAPad]
+    // (the near-field window reuses the A chunk region and what follows)
+    float* wl = reinterpret_cast<float*>(smem);
+    float* tap = wl + P.ns;
+    float* a_lds = tap + ((nq + 3) & ~3);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r0 = blockIdx.x * kTM;
@@ -92,6 +95,22 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
         const bool near = q >= -kNear && q < kNear;  // evaluated in the epilogue
         tap[e] = (m >= -P.ns && m <= P.ns && !near) ? P.taps[P.ns + m] : 0.0f;
     }
+    for (int e = tid; e < P.ns; e += 256) wl[e] = wt(P, e);
+
+    // A chunk loads (unweighted), issued one chunk ahead of the MFMA work
+    constexpr int kPer = kTM * kKC / 256;
+    float v[kPer];
+    auto load_chunk = [&](int i0) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = tid + u * 256;
+            const int rr = e / kKC, ii = e % kKC;
+            const int r = r0 + rr, i = i0 + ii;
+            v[u] = (r < P.nrow && i < kdim)
+                       ? P.x[static_cast<size_t>(r) * P.ns + 2 * i + kpar] : 0.0f;
+        }
+    };
+    load_chunk(0);
 
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // wave block in the tile
     f32x4 acc[2][2];
@@ -101,27 +120,17 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
         for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
     for (int i0 = 0; i0 < kdim; i0 += kKC) {
-        __syncthreads();  // taps written / previous chunk consumed
-        // Stage A: rows r0.., inputs i0.. (x[r][2i + kpar] * w), 16 per thread.
-        float v[kTM * kKC / 256];
+        __syncthreads();  // taps / weights written, previous chunk consumed
+        // Stage A: rows r0.., inputs i0.. (x[r][2i + kpar] * w[2i + kpar]).
 #pragma unroll
-        for (int u = 0; u < kTM * kKC / 256; ++u) {
+        for (int u = 0; u < kPer; ++u) {
             const int e = tid + u * 256;
-            const int rr = e / kKC, ii = e % kKC;
-            const int r = r0 + rr, i = i0 + ii;
-            float val = 0.0f;
-            if (r < P.nrow && i < kdim) {
-                const int k = 2 * i + kpar;
-                val = P.x[static_cast<size_t>(r) * P.ns + k] * wt(P, k);
-            }
-            v[u] = val;
-        }
-#pragma unroll
-        for (int u = 0; u < kTM * kKC / 256; ++u) {
-            const int e = tid + u * 256;
-            a_lds[(e / kKC) * (kKC + kAPad) + e % kKC] = v[u];
+            const int ii = e % kKC, i = i0 + ii;
+            const float w = i < kdim ? wl[2 * i + kpar] : 0.0f;
+            a_lds[(e / kKC) * (kKC + kAPad) + ii] = v[u] * w;
         }
         __syncthreads();
+        if (i0 + kKC < kdim) load_chunk(i0 + kKC);  // in flight during the MFMAs
         const int tapbase = j0 + wc - i0 + pi - 1 - qmin;  // G[i][j] = ho[j - i + pi - 1]
         if constexpr (!BF16X3) {
 #pragma unroll 4
@@ -186,7 +195,7 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
         float val = 0.0f;
         if (r < P.nrow && i >= 0 && i < kdim) {
             const int k = 2 * i + kpar;
-            val = P.x[static_cast<size_t>(r) * P.ns + k] * wt(P, k);
+            val = P.x[static_cast<size_t>(r) * P.ns + k] * wl[k];
         }
         win[e] = val;
     }
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
                 if (r < P.nrow && j < nout) {
                     const int n = 2 * j + pi;
                     const size_t o = static_cast<size_t>(r) * P.ns + n;
-                    const float xw = P.x[o] * wt(P, n);
+                    const float xw = P.x[o] * wl[n];
                     // input i = j + pi - 1 - q sits at window slot jj + kNear - 1 - q
                     const float* wrow = win + rr * kWin + jj + kNear - 1;
                     double near = h0 * static_cast<double>(xw);
@@ -264,8 +273,9 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
         P.n_odd = ns / 2;
         P.tiles_even = div_up(P.n_even, kTN);
         P.ds = g.ds;
-        const size_t lds = std::max(static_cast<size_t>(kTM) * (kKC + kAPad) + ns + kKC + kTN + 10,
-                                    static_cast<size_t>(kTM) * kWin) * sizeof(float);
+        const size_t nq4 = (static_cast<size_t>(ns) + kKC + kTN + 10 + 3) & ~static_cast<size_t>(3);
+        const size_t lds = (ns + nq4 + std::max(static_cast<size_t>(kTM) * (kKC + kAPad),
+                                                static_cast<size_t>(kTM) * kWin)) * sizeof(float);
         CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "Toeplitz ramp filter: ns too large");
         auto kern = bf16x3 ? ramp_mfma_kernel<true> : ramp_mfma_kernel<false>;
         if (lds > 64 * 1024)
