@@ -123,10 +123,7 @@ __global__ __launch_bounds__(256) void fp_kernel(const T* __restrict__ img,
 // ds_read2st64_b32.  Each ray keeps the reference's FP64 position
 // accumulation (x += offset per row) and inclusion test; the lerp and the
 // sum are FP32.  Views of the other branch in the group take a second pass.
-constexpr int kFpRows = 4;
-constexpr int kFpSpan = 512;  // staged columns per row (host checks the bound)
-
-template <int NB, int V>
+template <int NB, int V, int kFpRows, int kFpSpan>
 __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ img,
                                                      const float* __restrict__ imgT,
                                                      float* __restrict__ proj,
@@ -218,23 +215,23 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (has_ray) {
+                // right tap through an offset the compiler cannot fold, so the
+                // slice pairs load as ds_read2st64_b32 (rows kFpSpan apart); at
+                // x == hi its weight u is 0 and it reads a staged zero
+                int one = 1;
+                asm volatile("" : "+v"(one));
 #pragma unroll 1
                 for (int kk = 0; kk < nr; ++kk) {
                     const float* rk = rows + kk * NB * kFpSpan - cmin;
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
                         if (act[v] && x[v] >= 0 && x[v] <= hi) {
-                            // x >= 0: truncation is the floor
+                            // x >= 0: truncation is the floor, and the fraction is exact
                             const int i1 = static_cast<int>(x[v]);
-                            const float u = static_cast<float>(x[v] - static_cast<double>(i1));
+                            const float u = static_cast<float>(__builtin_amdgcn_fract(x[v]));
                             const float w0 = 1.0f - u;
-                            // right tap through a base the compiler cannot fold, so the
-                            // slice pairs load as ds_read2st64_b32 (rows kFpSpan apart);
-                            // at x == hi its weight u is 0 and it reads a staged zero
-                            int one = 1;
-                            asm volatile("" : "+v"(one));
                             const float* q0 = rk + i1;
-                            const float* q1 = rk + i1 + one;
+                            const float* q1 = q0 + one;
 #pragma unroll
                             for (int b = 0; b < NB; ++b)
                                 sum[v][b] = fmaf(u, q1[b * kFpSpan], fmaf(w0, q0[b * kFpSpan], sum[v][b]));
@@ -404,25 +401,30 @@ int launch_fp(const FpProblem& P, const T* img, const T* imgT, T* proj, const do
 
 constexpr int kFpViews = 2;
 constexpr int kFpSlices = 4;
+// 4-row chunks with 512-column spans (32 KB of LDS at 4 slices, 5 workgroups
+// per CU) measured 0.88 ms on the 8 x 512^2, 720-view scan against 0.95 ms
+// for 8-row chunks with 384-column spans (48 KB, 3 workgroups per CU).
+constexpr int kFpRowsDefault = 4, kFpSpanDefault = 512;
 
-// Columns the rays of one fp_lds_kernel chunk can span: 256 channels at up
+// Columns the rays of one fp_lds_kernel chunk can span: cpb channels at up
 // to sqrt(2) ds / pixel per channel, kFpRows rows at up to one column per row
 // (in pixel units of the marched axis), and the drift of a ray's column
 // across kFpViews views (at most 4 R |orbit| per view for an image of
 // half-diagonal R pixels), plus margins.
-bool fp_lds_applies(const ctr_geom& g)
+bool fp_lds_applies(const ctr_geom& g, int rows = kFpRowsDefault, int span_max = kFpSpanDefault)
 {
     const double pmin = std::min(std::fabs(g.dx), std::fabs(g.dy));
     const double pmax = std::max(std::fabs(g.dx), std::fabs(g.dy));
     if (!(pmin > 0) || !(g.ds > 0)) return false;
     const double r = 0.5 * std::hypot(g.nx + 2 * std::fabs(g.offset_x),
                                       g.ny + 2 * std::fabs(g.offset_y)) * pmax / pmin;
-    const double span = 255.0 * std::sqrt(2.0) * g.ds / pmin + kFpRows * pmax / pmin +
+    const int cpb = div_up(g.ns, div_up(g.ns, 256));  // channels per workgroup
+    const double span = (cpb - 1) * std::sqrt(2.0) * g.ds / pmin + rows * pmax / pmin +
                         4.0 * r * kFpViews * std::fabs(g.orbit) + 8.0;
-    return span < kFpSpan - 64;
+    return span + 2.0 <= span_max;
 }
 
-template <int NB>
+template <int NB, int R = kFpRowsDefault, int SPAN = kFpSpanDefault>
 int launch_fp_lds(const FpProblem& P, const float* img, const float* imgT, float* proj,
                   const double* sc, const double* bo, const int* usex, hipStream_t st)
 {
@@ -430,8 +432,8 @@ int launch_fp_lds(const FpProblem& P, const float* img, const float* imgT, float
     const int nblk = div_up(P.ns, 256);
     const int cpb = div_up(P.ns, nblk);
     dim3 grid(nblk, div_up(P.na, kFpViews), div_up(P.nbatch, NB));
-    hipLaunchKernelGGL((fp_lds_kernel<NB, kFpViews>), grid, dim3(256), 0, st, img, imgT, proj, sc,
-                       bo, usex, P, cpb);
+    hipLaunchKernelGGL((fp_lds_kernel<NB, kFpViews, R, SPAN>), grid, dim3(256), 0, st, img, imgT,
+                       proj, sc, bo, usex, P, cpb);
     return check_launch("fp_lds_kernel");
 }
 

@@ -56,6 +56,7 @@ struct ToeplitzParams {
     const double* taps64;  // FP64 h[m] at taps64[ns + m] (centre and near field)
     const float* taps;   // FP32 h[m] at taps[ns + m], m in [-ns, ns]
     const float* wts;    // FP32 per-channel weight, or nullptr
+    const __bf16* bcopies;  // BF16x3: far-field tap copies (bf16_tap_copies_kernel)
     int ns, nrow;
     int n_even, n_odd;   // outputs of each parity
     int tiles_even;      // column tiles of parity 0 (the rest are parity 1)
@@ -65,6 +66,66 @@ struct ToeplitzParams {
 __device__ __forceinline__ float wt(const ToeplitzParams& P, int k)
 {
     return P.wts ? P.wts[k] : 1.0f;
+}
+
+// Epilogue shared by the FP32 and BF16x3 kernels: stages the near-field
+// window of the tile's rows into win (after a barrier: the caller's operand
+// buffers are dead), then writes
+//   out[2j+pi] = ds (h0 xw[2j+pi] + sum_q h[2q+1] xw[2(j+pi-1-q)+kpar] + far)
+// with the centre and near taps in FP64 and far = the GEMM's FP32 result.
+__device__ __forceinline__ void near_field_epilogue(const ToeplitzParams& P, const float* wl,
+                                                    float* win, const f32x4 (&acc)[2][2], int r0,
+                                                    int j0, int pi, int kpar, int kdim, int nout,
+                                                    int wr, int wc, int tid, int lane)
+{
+    // Near-field window: win[rr][w] = weighted input of parity 1-pi at
+    // i = j0 + pi - kNear + w (the A operand's values), zero off the row.
+    __syncthreads();  // last chunk consumed: the window reuses operand LDS
+    const int iw0 = j0 + pi - kNear;
+    for (int e = tid; e < kTM * kWin; e += 256) {
+        const int rr = e / kWin, w = e % kWin;
+        const int r = r0 + rr, i = iw0 + w;
+        float val = 0.0f;
+        if (r < P.nrow && i >= 0 && i < kdim) {
+            const int k = 2 * i + kpar;
+            val = P.x[static_cast<size_t>(r) * P.ns + k] * wl[k];
+        }
+        win[e] = val;
+    }
+    __syncthreads();
+
+    // D layout: lane holds D[4*(lane/16) + t][lane%16], t = 0..3.
+    // out[2j+pi] = ds (h0 xw[2j+pi] + sum_q h[2q+1] xw[2(j+pi-1-q)+kpar] + far)
+    const double h0 = P.taps64[P.ns];
+    double hn[2 * kNear];
+#pragma unroll
+    for (int q = -kNear; q < kNear; ++q) {
+        const int m = 2 * q + 1;
+        hn[q + kNear] = (m >= -P.ns && m <= P.ns) ? P.taps64[P.ns + m] : 0.0;
+    }
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int rr = wr + 16 * ta + 4 * (lane >> 4) + t;
+                const int jj = wc + 16 * tb + (lane & 15);
+                const int r = r0 + rr, j = j0 + jj;
+                if (r < P.nrow && j < nout) {
+                    const int n = 2 * j + pi;
+                    const size_t o = static_cast<size_t>(r) * P.ns + n;
+                    const float xw = P.x[o] * wl[n];
+                    // input i = j + pi - 1 - q sits at window slot jj + kNear - 1 - q
+                    const float* wrow = win + rr * kWin + jj + kNear - 1;
+                    double near = h0 * static_cast<double>(xw);
+#pragma unroll
+                    for (int q = -kNear; q < kNear; ++q)
+                        near = fma(hn[q + kNear], static_cast<double>(wrow[-q]), near);
+                    P.out[o] = static_cast<float>(
+                        P.ds * (near + static_cast<double>(acc[ta][tb][t])));
+                }
+            }
 }
 
 template <bool BF16X3>
@@ -184,55 +245,157 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
         }
     }
 
-    // Near-field window: win[rr][w] = weighted input of parity 1-pi at
-    // i = j0 + pi - kNear + w (the A operand's values), zero off the row.
-    __syncthreads();  // last chunk consumed: reuse the A buffer
-    float* win = a_lds;  // [kTM][kWin]: the launch sizes LDS for it
-    const int iw0 = j0 + pi - kNear;
-    for (int e = tid; e < kTM * kWin; e += 256) {
-        const int rr = e / kWin, w = e % kWin;
-        const int r = r0 + rr, i = iw0 + w;
-        float val = 0.0f;
-        if (r < P.nrow && i >= 0 && i < kdim) {
-            const int k = 2 * i + kpar;
-            val = P.x[static_cast<size_t>(r) * P.ns + k] * wl[k];
-        }
-        win[e] = val;
-    }
-    __syncthreads();
+    near_field_epilogue(P, wl, a_lds, acc, r0, j0, pi, kpar, kdim, nout, wr, wc, tid, lane);
+}
 
-    // D layout: lane holds D[4*(lane/16) + t][lane%16], t = 0..3.
-    // out[2j+pi] = ds (h0 xw[2j+pi] + sum_q h[2q+1] xw[2(j+pi-1-q)+kpar] + far)
-    const double h0 = P.taps64[P.ns];
-    double hn[2 * kNear];
-#pragma unroll
-    for (int q = -kNear; q < kNear; ++q) {
-        const int m = 2 * q + 1;
-        hn[q + kNear] = (m >= -P.ns && m <= P.ns) ? P.taps64[P.ns + m] : 0.0;
+// BF16x3 kernel with the operand splits precomputed in LDS.  A chunk is
+// stored once per chunk as hi / lo BF16 rows (stride kAS, 16-byte rows), so a
+// lane's 8 consecutive K values are one ds_read_b128.  A lane's 8 Toeplitz
+// values are 8 consecutive taps in decreasing order; the far-field tap
+// vector is stored reversed (hi / lo BF16) in 8 copies shifted by one entry,
+// and the copy matching the window's alignment gives them as one aligned
+// ds_read_b128.  Per K-step each wave issues 8 such reads for 12 MFMAs
+// (2 x 2 tiles x hi*hi, hi*lo, lo*hi).
+constexpr int kAS = kKC + 8;  // bf16 row stride of the A splits (144 B)
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// Entries per tap copy (a multiple of 8 so every copy starts 16-byte aligned).
+__host__ __device__ __forceinline__ int rev_len(int nq) { return ((nq + 7) & ~7) + 8; }
+
+// NC tap copies: 8 (each fragment one ds_read_b128) or 4 (two ds_read_b64,
+// half the LDS).
+template <int NC>
+__global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int qmin = -(P.ns / 2) - kKC - 4;
+    const int nq = P.ns + kKC + kTN + 10;
+    const int nrev = rev_len(nq);
+    // LDS: weights [ns] | B copies hi/lo [8][nrev] bf16 (the near window reuses
+    // them after the K loop) | A splits hi/lo [kTM][kAS] bf16
+    float* wl = reinterpret_cast<float*>(smem);
+    unsigned char* bbase = smem + ((static_cast<size_t>(P.ns) * 4 + 15) & ~static_cast<size_t>(15));
+    __bf16* bhi = reinterpret_cast<__bf16*>(bbase);
+    __bf16* blo = bhi + NC * nrev;
+    const size_t bbytes = std::max(static_cast<size_t>(2 * NC) * nrev * 2,
+                                   static_cast<size_t>(kTM) * kWin * 4);
+    __bf16* ahi = reinterpret_cast<__bf16*>(bbase + ((bbytes + 15) & ~static_cast<size_t>(15)));
+    __bf16* alo = ahi + kTM * kAS;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = blockIdx.x * kTM;
+    const int pi = blockIdx.y < P.tiles_even ? 0 : 1;
+    const int j0 = (pi == 0 ? blockIdx.y : blockIdx.y - P.tiles_even) * kTN;
+    const int nout = pi == 0 ? P.n_even : P.n_odd;
+    const int kdim = pi == 0 ? P.ns / 2 : (P.ns + 1) / 2;
+    const int kpar = 1 - pi;
+
+    // the tap copies, built once per call by bf16_tap_copies_kernel in this
+    // layout, arrive as 16-byte loads
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(P.bcopies);
+        uint4* dst = reinterpret_cast<uint4*>(bbase);
+        for (int e = tid; e < NC * nrev / 4; e += 256) dst[e] = src[e];  // 2 NC nrev bf16
     }
+    for (int e = tid; e < P.ns; e += 256) wl[e] = wt(P, e);
+
+    constexpr int kPer = kTM * kKC / 256;
+    float v[kPer];
+    auto load_chunk = [&](int i0) {
 #pragma unroll
-    for (int ta = 0; ta < 2; ++ta)
+        for (int u = 0; u < kPer; ++u) {
+            const int e = tid + u * 256;
+            const int rr = e / kKC, ii = e % kKC;
+            const int r = r0 + rr, i = i0 + ii;
+            v[u] = (r < P.nrow && i < kdim)
+                       ? P.x[static_cast<size_t>(r) * P.ns + 2 * i + kpar] : 0.0f;
+        }
+    };
+    load_chunk(0);
+
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    f32x4 acc[2][2];
 #pragma unroll
-        for (int tb = 0; tb < 2; ++tb)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int rr = wr + 16 * ta + 4 * (lane >> 4) + t;
-                const int jj = wc + 16 * tb + (lane & 15);
-                const int r = r0 + rr, j = j0 + jj;
-                if (r < P.nrow && j < nout) {
-                    const int n = 2 * j + pi;
-                    const size_t o = static_cast<size_t>(r) * P.ns + n;
-                    const float xw = P.x[o] * wl[n];
-                    // input i = j + pi - 1 - q sits at window slot jj + kNear - 1 - q
-                    const float* wrow = win + rr * kWin + jj + kNear - 1;
-                    double near = h0 * static_cast<double>(xw);
+        for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int i0 = 0; i0 < kdim; i0 += kKC) {
+        __syncthreads();  // B copies / weights written, previous chunk consumed
 #pragma unroll
-                    for (int q = -kNear; q < kNear; ++q)
-                        near = fma(hn[q + kNear], static_cast<double>(wrow[-q]), near);
-                    P.out[o] = static_cast<float>(
-                        P.ds * (near + static_cast<double>(acc[ta][tb][t])));
+        for (int u = 0; u < kPer; ++u) {
+            const int e = tid + u * 256;
+            const int rr = e / kKC, ii = e % kKC, i = i0 + ii;
+            const float a = v[u] * (i < kdim ? wl[2 * i + kpar] : 0.0f);
+            const __bf16 h = static_cast<__bf16>(a);
+            ahi[rr * kAS + ii] = h;
+            alo[rr * kAS + ii] = static_cast<__bf16>(a - static_cast<float>(h));
+        }
+        __syncthreads();
+        if (i0 + kKC < kdim) load_chunk(i0 + kKC);  // in flight during the MFMAs
+        const int tapbase = j0 + wc - i0 + pi - 1 - qmin;  // G[i][j] = ho[j - i + pi - 1]
+#pragma unroll
+        for (int ik = 0; ik < kKC; ik += 32) {
+            // lane: A[m = lane%16][k = 8(lane/16) + e], B[k = 8(lane/16) + e][n = lane%16]
+            const int m = lane & 15, kb = ik + 8 * (lane >> 4);
+            bf16x8 fa_hi[2], fa_lo[2], fb_hi[2], fb_lo[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int ro = (wr + 16 * t + m) * kAS + kb;
+                fa_hi[t] = *reinterpret_cast<const bf16x8*>(ahi + ro);
+                fa_lo[t] = *reinterpret_cast<const bf16x8*>(alo + ro);
+                // taps Q, Q-1, ..., Q-7 = reversed entries S .. S+7, S = nq-1-Q
+                const int S = nq - 1 - (tapbase + 16 * t + m - kb);
+                const int c = S & (NC - 1), bo = c * nrev + (S - c);
+                if constexpr (NC == 8) {
+                    fb_hi[t] = *reinterpret_cast<const bf16x8*>(bhi + bo);
+                    fb_lo[t] = *reinterpret_cast<const bf16x8*>(blo + bo);
+                } else {
+                    const bf16x4 h0 = *reinterpret_cast<const bf16x4*>(bhi + bo);
+                    const bf16x4 h1 = *reinterpret_cast<const bf16x4*>(bhi + bo + 4);
+                    const bf16x4 l0 = *reinterpret_cast<const bf16x4*>(blo + bo);
+                    const bf16x4 l1 = *reinterpret_cast<const bf16x4*>(blo + bo + 4);
+                    fb_hi[t] = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
+                    fb_lo[t] = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
                 }
             }
+#pragma unroll
+            for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+                for (int tb = 0; tb < 2; ++tb) {
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_lo[ta], fb_hi[tb],
+                                                                          acc[ta][tb], 0, 0, 0);
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi[ta], fb_lo[tb],
+                                                                          acc[ta][tb], 0, 0, 0);
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi[ta], fb_hi[tb],
+                                                                          acc[ta][tb], 0, 0, 0);
+                }
+        }
+    }
+    near_field_epilogue(P, wl, reinterpret_cast<float*>(bbase), acc, r0, j0, pi, kpar, kdim, nout,
+                        wr, wc, tid, lane);
+}
+
+// The BF16x3 kernel's tap operand: copy c (c < nc), entry j holds the
+// reversed far-field tap nq-1-(j+c) split as hi (first nc nrev entries) and
+// lo (next nc nrev entries), i.e. exactly its LDS image.
+__global__ void bf16_tap_copies_kernel(const float* __restrict__ hf, __bf16* __restrict__ out,
+                                       int ns, int nq, int nrev, int qmin, int nc)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nc * nrev) return;
+    const int c = e / nrev, j = e - c * nrev;
+    const int tj = nq - 1 - (j + c);
+    float f = 0.0f;
+    if (tj >= 0) {
+        const int q = tj + qmin, m = 2 * q + 1;
+        const bool near = q >= -kNear && q < kNear;
+        f = (m >= -ns && m <= ns && !near) ? hf[ns + m] : 0.0f;
+    }
+    const __bf16 h = static_cast<__bf16>(f);
+    out[e] = h;
+    out[nc * nrev + e] = static_cast<__bf16>(f - static_cast<float>(h));
 }
 
 // FP32 taps and (optionally) per-channel cosine weights from the FP64 taps.
@@ -249,17 +412,36 @@ __global__ void toeplitz_prep_kernel(const double* __restrict__ h, float* __rest
     }
 }
 
+// Tap copies of the BF16x3 kernel: 4 (two ds_read_b64 per fragment, 50 KB of
+// LDS at 1472 channels) measured 0.33 ms on 8 x 1440 rows of 1472 against
+// 0.42 ms for 8 copies (one ds_read_b128, 76 KB: two workgroups per CU).
+constexpr int kBf16Copies = 4;
+static int bf16_copies() { return kBf16Copies; }
+
 int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int nrow,
                   const double* filter, int weight_type, bool bf16x3, hipStream_t st)
 {
     CTR_REQUIRE(g.ns >= 2, CTR_ERR_ARG, "Toeplitz ramp filter: ns must be >= 2");
     const int ns = static_cast<int>(g.ns);
-    float* hf = nullptr;
-    CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&hf), (3 * ns + 1) * sizeof(float), st));
+    const int nq = ns + kKC + kTN + 10;
+    const int nrev = ((nq + 7) & ~7) + 8;
+    // scratch: FP32 taps [2ns+1] | weights [ns] | (BF16x3) tap copies [16 nrev] bf16
+    const size_t fbytes = ((3 * static_cast<size_t>(ns) + 1) * sizeof(float) + 15) & ~static_cast<size_t>(15);
+    const int nc = bf16_copies();
+    const size_t cbytes = bf16x3 ? static_cast<size_t>(2 * nc) * nrev * sizeof(__bf16) : 0;
+    unsigned char* scratch = nullptr;
+    CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&scratch), fbytes + cbytes, st));
+    float* hf = reinterpret_cast<float*>(scratch);
     float* w = weight_type ? hf + 2 * ns + 1 : nullptr;
+    __bf16* copies = bf16x3 ? reinterpret_cast<__bf16*>(scratch + fbytes) : nullptr;
     hipLaunchKernelGGL(toeplitz_prep_kernel, dim3(div_up(2 * ns + 1, 256)), dim3(256), 0, st,
                        filter, hf, w, ns, weight_type, g.ds, g.dso, g.dsd, cent_s(g));
     int rc = check_launch("toeplitz_prep_kernel");
+    if (rc == CTR_OK && bf16x3) {
+        hipLaunchKernelGGL(bf16_tap_copies_kernel, dim3(div_up(nc * nrev, 256)), dim3(256), 0, st,
+                           hf, copies, ns, nq, nrev, -(ns / 2) - kKC - 4, nc);
+        rc = check_launch("bf16_tap_copies_kernel");
+    }
     if (rc == CTR_OK) {
         ToeplitzParams P{};
         P.x = in;
@@ -267,6 +449,7 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
         P.taps64 = filter;
         P.taps = hf;
         P.wts = w;
+        P.bcopies = copies;
         P.ns = ns;
         P.nrow = static_cast<int>(nrow);
         P.n_even = (ns + 1) / 2;
@@ -274,10 +457,17 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
         P.tiles_even = div_up(P.n_even, kTN);
         P.ds = g.ds;
         const size_t nq4 = (static_cast<size_t>(ns) + kKC + kTN + 10 + 3) & ~static_cast<size_t>(3);
-        const size_t lds = (ns + nq4 + std::max(static_cast<size_t>(kTM) * (kKC + kAPad),
-                                                static_cast<size_t>(kTM) * kWin)) * sizeof(float);
+        size_t lds = (ns + nq4 + std::max(static_cast<size_t>(kTM) * (kKC + kAPad),
+                                          static_cast<size_t>(kTM) * kWin)) * sizeof(float);
+        if (bf16x3) {
+            const size_t bbytes = std::max(static_cast<size_t>(2 * nc) * nrev * 2,
+                                           static_cast<size_t>(kTM) * kWin * 4);
+            lds = ((static_cast<size_t>(ns) * 4 + 15) & ~static_cast<size_t>(15)) +
+                  ((bbytes + 15) & ~static_cast<size_t>(15)) +
+                  static_cast<size_t>(2) * kTM * kAS * 2;
+        }
         CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "Toeplitz ramp filter: ns too large");
-        auto kern = bf16x3 ? ramp_mfma_kernel<true> : ramp_mfma_kernel<false>;
+        auto kern = bf16x3 ? ramp_bf16x3_kernel<kBf16Copies> : ramp_mfma_kernel<false>;
         if (lds > 64 * 1024)
             CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -286,7 +476,7 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
         hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, P);
         rc = check_launch("ramp_mfma_kernel");
     }
-    CTR_HIP(hipFreeAsync(hf, st));
+    CTR_HIP(hipFreeAsync(scratch, st));
     return rc;
 }
 
