@@ -106,6 +106,10 @@ def cpu_baseline(cfg, sinos, threads, budget_s=30.0):
     def fbp(x):
         return O.fan_bp(g, O.ramp(g, O.fanw(g, x, kind=kind), taps, kind=kind), tables, kind=kind)
 
+    # single thread, one slice: the reference's own execution (a serial loop)
+    t0 = time.perf_counter()
+    fbp(sinos[0])
+    single_s = time.perf_counter() - t0
     jobs = [sinos[0], sinos[0].astype(np.float64)]
     jobs += [sinos[b % len(sinos)] for b in range(1, threads - 1)]
     t0 = time.perf_counter()
@@ -114,7 +118,7 @@ def cpu_baseline(cfg, sinos, threads, budget_s=30.0):
     wall = time.perf_counter() - t0
     pv = len(jobs) * cfg["n"] ** 2 * cfg["na"]
     return dict(kind="reference" if kind == "ref" else "port", wall_s=wall, slices=len(jobs),
-                pixel_views=pv, ref_f32=outs[0], ref_f64=outs[1])
+                pixel_views=pv, ref_f32=outs[0], ref_f64=outs[1], single_s=single_s)
 
 
 TRAFFIC_PROFILE = "profiles/fan_bp_16x512sq_720v.md"
@@ -150,7 +154,7 @@ def profile_kernels(torch, F, which):
         proj = torch.randn((cfg["nbatch"], cfg["na"], cfg["ns"]), generator=g, device="cuda")
         for _ in range(3):
             img = (F.fan_backprojection_2d(proj, **kw_fan(cfg)) if which == "bp"
-                   else F.fan_filter(proj, **kw_filter(cfg), impl="mfma"))
+                   else F.fan_filter(proj, **kw_filter(cfg)))
     torch.cuda.synchronize()
     src = torch.randn(64 * 1024 * 1024, generator=g, device="cuda")
     evict = torch.zeros(256 * 1024 * 1024, device="cuda")
@@ -160,6 +164,60 @@ def profile_kernels(torch, F, which):
     torch.cuda.synchronize()
     print(json.dumps({"profile": which, "checksum": float(img.double().abs().sum()),
                       "copy_bytes": src.numel() * 4, "evict_bytes": evict.numel() * 4}))
+
+
+C1_SEED = 20261206
+
+
+def bench_c1(torch, F, threads, with_cpu=True):
+    """Plain parallel-beam FBP of a 256^2 seeded 10-ellipse phantom: ray-driven
+    FP to 360 views over 180 deg, 367 channels (ds = dx = 1), Ram-Lak, then
+    pixel-driven BP -- on the GPU, and the reference CPU path on one thread
+    and on `threads` threads (one copy per thread)."""
+    from ct_recon_extensions_amd import phantom as P
+    from oracle import oracle as O
+    n, na, ns = 256, 360, 367
+    img = P.rasterize(P.random_ellipses(10, C1_SEED, radius=n / 2), n, n, 1.0, 1.0)
+    img = img.astype(np.float32)
+    kw = dict(img_shape=[n, n], img_space=[1.0, 1.0], proj_shape=[na, ns], channel_space=1.0,
+              orbit=math.pi / na)
+    x = torch.from_numpy(img)[None].cuda()
+
+    def gpu():
+        s = F.parallel_projection_2d(x, **kw)
+        f = F.ramp_filter(s, ns, na, 1.0, type="par")
+        return F.parallel_backprojection_2d(f, **kw)
+
+    out = gpu()
+    torch.cuda.synchronize()
+    ms = time_events(gpu, 20, torch)
+    res = {"workload": "256^2 phantom, FP 360 views/180deg, 367 ch, Ram-Lak, parallel BP",
+           "seed": C1_SEED, "gpu_ms": round(ms, 4)}
+    if with_cpu:
+        kind = "ref" if O.have_ref() else "oracle"
+        g = O.geom(ns=ns, na=na, ds=1.0, orbit=math.pi / na, nx=n, ny=n, dx=1.0, dy=1.0,
+                   type=0, nrow=na)
+        tf, tp, taps = O.fp_prep(g, kind), O.par_prep(g, kind), O.ramp_prep(g, kind)
+
+        def cpu(v):
+            return O.par_bp(g, O.ramp(g, O.fp(g, v, tf, kind=kind), taps, kind=kind), tp,
+                            kind=kind)
+
+        t0 = time.perf_counter()
+        ref = cpu(img)
+        one = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            list(ex.map(cpu, [img] * threads))
+        allc = time.perf_counter() - t0
+        got = out[0].double().cpu().numpy()
+        res.update({"cpu_kind": "reference" if kind == "ref" else "port",
+                    "cpu_1thread_ms": round(one * 1e3, 2),
+                    "cpu_all_threads_ms_per_slice": round(allc * 1e3 / threads, 2),
+                    "cpu_threads": threads,
+                    "rel_l2_vs_reference_f32": float(np.linalg.norm(got - ref) /
+                                                     np.linalg.norm(ref))})
+    return res
 
 
 def bench_lsq(torch, steps):
@@ -281,12 +339,13 @@ def main():
     ap.add_argument("--lsq-steps", type=int, default=200)
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batch")
     ap.add_argument("--no-precision", action="store_true", help="skip the filter precision study")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 256^2 parallel FBP")
     ap.add_argument("--profile-bp", action="store_true",
                     help="counter-collection mode: three fan BP launches of the workload, then a "
                          "256 MiB device copy from a cold source (calibrates FETCH/WRITE_SIZE)")
     ap.add_argument("--profile", choices=("bp", "filter", "lsq"), default=None,
                     help="counter-collection mode for one kernel family: 'bp' as --profile-bp; "
-                         "'filter' three fused fan-weighted ramp filters (FP32 MFMA) of the "
+                         "'filter' three fused fan-weighted ramp filters (default implementation) of the "
                          "workload; 'lsq' three FP and FP^T launches of the least-squares "
                          "workload; each followed by the calibration copy")
     args = ap.parse_args()
@@ -364,9 +423,35 @@ def main():
         torch.cuda.synchronize()
         ms_plain = time_events(plain_bp, 2, torch)
         plain_err = float((out_plain - img).double().norm() / out_plain.double().norm())
+        # the whole FBP with the same structure: per slice, fan weighting, the
+        # direct filter and the plain BP, the stream synchronised after each
+        def plain_fbp():
+            for b in range(B):
+                x = F.fan_weighting(proj[b:b + 1], proj_shape=[na, ns], channel_space=cfg["ds"],
+                                    dso=cfg["dso"], dsd=cfg["dsd"], tp="fan")
+                torch.cuda.synchronize()
+                f = F.ramp_filter(x, ns, na, cfg["ds"], cfg["dsd"], "fan", impl="direct")
+                torch.cuda.synchronize()
+                _lib.check(lib.ctr_plain_fan_bp(C.byref(g), 0, C.c_void_p(f.data_ptr()),
+                                                C.c_void_p(out_plain[b].data_ptr()), 1,
+                                                C.c_void_p(xpos.data_ptr()),
+                                                C.c_void_p(ypos.data_ptr()),
+                                                C.c_void_p(sc.data_ptr()), p.scale, st), "plain")
+
+        plain_fbp()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain_fbp()
+        torch.cuda.synchronize()
+        ms_plain_fbp = (time.perf_counter() - t0) * 1e3
+        plain_fbp_err = float((out_plain - img).double().norm() / out_plain.double().norm())
         plain = dict(ms_per_batch_bp=ms_plain, slices_per_s_bp=B / (ms_plain * 1e-3),
-                     speedup_bp=ms_plain / ms_bp, rel_l2_fast_vs_plain=plain_err)
-        print(f"[bench] plain BP {ms_plain:.2f} ms/batch", file=sys.stderr, flush=True)
+                     speedup_bp=ms_plain / ms_bp, rel_l2_fast_vs_plain=plain_err,
+                     ms_per_batch_fbp=ms_plain_fbp, slices_per_s_fbp=B / (ms_plain_fbp * 1e-3),
+                     speedup_fbp_slices_per_s=ms_plain_fbp / ms_step,
+                     rel_l2_fast_vs_plain_fbp=plain_fbp_err)
+        print(f"[bench] plain BP {ms_plain:.2f} ms/batch, plain FBP {ms_plain_fbp:.2f} ms/batch",
+              file=sys.stderr, flush=True)
 
     result = {
         "metric": "fan-beam FBP backprojection throughput",
@@ -415,6 +500,11 @@ def main():
         print(f"[bench] ragged {result['ragged']['ms_per_batch']:.3f} ms", file=sys.stderr,
               flush=True)
 
+    if not args.no_c1:
+        result["c1_parallel_fbp"] = bench_c1(torch, F, args.cpu_threads or min(16, os.cpu_count() or 1),
+                                             with_cpu=not args.no_cpu)
+        print(f"[bench] c1 parallel FBP {result['c1_parallel_fbp']['gpu_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
     if not args.no_precision:
         result["filter_precision"] = bench_filter_precision(torch, args.steps)
         fp_ = result["filter_precision"]
@@ -436,6 +526,8 @@ def main():
             "sample": f"{cb['slices']} full slices of the same workload (FBP incl. weighting "
                       f"and ramp), one per thread, {cb['wall_s']:.1f}s wall",
             "host_cpu": _cpu_model(), "nproc": os.cpu_count(),
+            "single_thread_value": round(cfg["n"] ** 2 * cfg["na"] / cb["single_s"] / 1e9, 5),
+            "single_thread_sample": f"1 slice on 1 thread, {cb['single_s']:.1f}s",
         }
         result["vs_cpu"] = round(value / cpu_rate, 1)
         result["rel_l2_vs_reference_f32"] = e32

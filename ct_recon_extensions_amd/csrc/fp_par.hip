@@ -351,41 +351,48 @@ __global__ __launch_bounds__(256) void fpt_kernel(const T* __restrict__ proj,
         if (b < nb) img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix] = static_cast<T>(sum[b]);
 }
 
-// The border pixels (nx, ny >= 3) after the engine pass: one workgroup per
-// border pixel and slice group, the views split over its threads and the
-// sums reduced in the workgroup.
+// The border pixels (nx, ny >= 3) after the engine pass, in two steps so
+// that neighbouring threads read neighbouring channels: (1) thread = border
+// pixel, a chunk of kBorderViews views per workgroup row, FP64 partial sums
+// per (view chunk, slice, pixel); (2) the partials added in chunk order.
+constexpr int kBorderViews = 16;
+
 template <int NB>
-__global__ __launch_bounds__(256) void fpt_border_kernel(const float* __restrict__ proj,
-                                                         float* __restrict__ img,
-                                                         const double* __restrict__ weights,
-                                                         const double* __restrict__ pos,
-                                                         const int* __restrict__ usex,
-                                                         FpProblem P, int stride)
+__global__ __launch_bounds__(256) void fpt_border_partial_kernel(
+    const float* __restrict__ proj, double* __restrict__ partial,
+    const double* __restrict__ weights, const double* __restrict__ pos,
+    const int* __restrict__ usex, FpProblem P, int stride, int nborder)
 {
-    __shared__ double red[4][NB];
-    int ix, iy;
-    border_pixel(blockIdx.x, P.nx, P.ny, ix, iy);
-    const int b0 = blockIdx.y * NB;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int vc = blockIdx.y;
+    const int b0 = blockIdx.z * NB;
     const int nb = min(NB, P.nbatch - b0);
+    if (i >= nborder) return;
+    int ix, iy;
+    border_pixel(i, P.nx, P.ny, ix, iy);
     double sum[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) sum[b] = 0.0;
-    for (int a = threadIdx.x; a < P.na; a += 256)
+    const int a1 = min(P.na, (vc + 1) * kBorderViews);
+    for (int a = vc * kBorderViews; a < a1; ++a)
         fpt_view<float, NB>(proj, weights, pos, usex, P, stride, a, ix, iy, b0, nb, sum);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-        for (int o = 32; o > 0; o >>= 1) sum[b] += __shfl_xor(sum[b], o);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) red[wave][b] = sum[b];
-    __syncthreads();
-    if (threadIdx.x < nb) {
-        const int b = threadIdx.x;
-        double t = 0.0;
-        for (int w = 0; w < 4; ++w) t += red[w][b];
-        img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix] = static_cast<float>(t);
-    }
+        if (b < nb) partial[(static_cast<size_t>(vc) * P.nbatch + b0 + b) * nborder + i] = sum[b];
+}
+
+__global__ void fpt_border_reduce_kernel(const double* __restrict__ partial,
+                                         float* __restrict__ img, FpProblem P, int nborder,
+                                         int nchunks)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= nborder) return;
+    int ix, iy;
+    border_pixel(i, P.nx, P.ny, ix, iy);
+    double t = 0.0;
+    for (int c = 0; c < nchunks; ++c) t += partial[(static_cast<size_t>(c) * P.nbatch + b) * nborder + i];
+    img[(static_cast<size_t>(b) * P.ny + iy) * P.nx + ix] = static_cast<float>(t);
 }
 
 namespace {
@@ -486,10 +493,22 @@ template <int NB>
 int launch_fpt_border(const FpProblem& P, const float* proj, float* img, const double* w,
                       const double* pos, const int* usex, int stride, hipStream_t st)
 {
-    dim3 grid(2 * P.nx + 2 * (P.ny - 2), div_up(P.nbatch, NB));
-    hipLaunchKernelGGL((fpt_border_kernel<NB>), grid, dim3(256), 0, st, proj, img, w, pos, usex,
-                       P, stride);
-    return check_launch("fpt_border_kernel");
+    const int nborder = 2 * P.nx + 2 * (P.ny - 2);
+    const int nchunks = div_up(P.na, kBorderViews);
+    double* partial = nullptr;
+    CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial),
+                           static_cast<size_t>(nchunks) * P.nbatch * nborder * sizeof(double), st));
+    hipLaunchKernelGGL((fpt_border_partial_kernel<NB>),
+                       dim3(div_up(nborder, 256), nchunks, div_up(P.nbatch, NB)), dim3(256), 0, st,
+                       proj, partial, w, pos, usex, P, stride, nborder);
+    int rc = check_launch("fpt_border_partial_kernel");
+    if (rc == CTR_OK) {
+        hipLaunchKernelGGL(fpt_border_reduce_kernel, dim3(div_up(nborder, 256), P.nbatch),
+                           dim3(256), 0, st, partial, img, P, nborder, nchunks);
+        rc = check_launch("fpt_border_reduce_kernel");
+    }
+    CTR_HIP(hipFreeAsync(partial, st));
+    return rc;
 }
 
 template <int NB>
