@@ -60,6 +60,13 @@ def _require_gpu(t: torch.Tensor, name: str):
         raise ValueError(f"{name} must be a GPU tensor (this library has no CPU path)")
 
 
+def _call_batched(name: str, n: int, *args) -> None:
+    """Call a batched C ABI operator; an empty batch is a no-op (the
+    reference's per-slice loop runs zero times)."""
+    if n > 0:
+        _lib.call(name, *args)
+
+
 def _to_batch(x: torch.Tensor, h: int, w: int, name: str):
     """View x as contiguous [N, h, w]; return it and a function restoring the
     caller's rank for an output of spatial shape (h2, w2)."""
@@ -88,8 +95,8 @@ def _fan_bp(proj3, p, tables, scale):
     out = torch.empty((n, p.ny, p.nx), dtype=proj3.dtype, device=proj3.device)
     g = p.ctr()
     xpos, ypos, sc = tables
-    _lib.call("ctr_fan_bp", C.byref(g), _dtype_code(proj3), _ptr(proj3), _ptr(out), n,
-              _ptr(xpos), _ptr(ypos), _ptr(sc), scale, _stream(proj3))
+    _call_batched("ctr_fan_bp", n, C.byref(g), _dtype_code(proj3), _ptr(proj3), _ptr(out), n,
+                  _ptr(xpos), _ptr(ypos), _ptr(sc), scale, _stream(proj3))
     return out
 
 
@@ -98,8 +105,9 @@ def _fan_bpt(img3, p, tables, scale, deterministic):
     out = torch.empty((n, p.na, p.ns), dtype=img3.dtype, device=img3.device)
     g = p.ctr()
     xpos, ypos, sc = tables
-    _lib.call("ctr_fan_bp_t", C.byref(g), _dtype_code(img3), _ptr(img3), _ptr(out), n,
-              _ptr(xpos), _ptr(ypos), _ptr(sc), scale, int(bool(deterministic)), _stream(img3))
+    _call_batched("ctr_fan_bp_t", n, C.byref(g), _dtype_code(img3), _ptr(img3), _ptr(out), n,
+                  _ptr(xpos), _ptr(ypos), _ptr(sc), scale, int(bool(deterministic)),
+                  _stream(img3))
     return out
 
 
@@ -217,8 +225,8 @@ class _ParBP(torch.autograd.Function):
         ctx.p, ctx.tables, ctx.det = p, tables, deterministic
         n = proj3.shape[0]
         out = torch.empty((n, p.ny, p.nx), dtype=proj3.dtype, device=proj3.device)
-        _lib.call("ctr_par_bp", C.byref(p.ctr()), _dtype_code(proj3), _ptr(proj3), _ptr(out), n,
-                  _ptr(tables[0]), _ptr(tables[1]), _stream(proj3))
+        _call_batched("ctr_par_bp", n, C.byref(p.ctr()), _dtype_code(proj3), _ptr(proj3),
+                      _ptr(out), n, _ptr(tables[0]), _ptr(tables[1]), _stream(proj3))
         return out
 
     @staticmethod
@@ -232,8 +240,9 @@ class _ParBPT(torch.autograd.Function):
         ctx.p, ctx.tables, ctx.det = p, tables, deterministic
         n = img3.shape[0]
         out = torch.empty((n, p.na, p.ns), dtype=img3.dtype, device=img3.device)
-        _lib.call("ctr_par_bp_t", C.byref(p.ctr()), _dtype_code(img3), _ptr(img3), _ptr(out), n,
-                  _ptr(tables[0]), _ptr(tables[1]), int(bool(deterministic)), _stream(img3))
+        _call_batched("ctr_par_bp_t", n, C.byref(p.ctr()), _dtype_code(img3), _ptr(img3),
+                      _ptr(out), n, _ptr(tables[0]), _ptr(tables[1]), int(bool(deterministic)),
+                      _stream(img3))
         return out
 
     @staticmethod
@@ -276,8 +285,8 @@ class _FP(torch.autograd.Function):
         n = img3.shape[0]
         out = torch.empty((n, p.na, p.ns), dtype=img3.dtype, device=img3.device)
         sc, bo, usex = t_fp
-        _lib.call("ctr_fp", C.byref(p.ctr()), _dtype_code(img3), _ptr(img3), _ptr(out), n,
-                  _ptr(sc), _ptr(bo), _ptr(usex), _stream(img3))
+        _call_batched("ctr_fp", n, C.byref(p.ctr()), _dtype_code(img3), _ptr(img3),
+                      _ptr(out), n, _ptr(sc), _ptr(bo), _ptr(usex), _stream(img3))
         return out
 
     @staticmethod
@@ -292,8 +301,9 @@ class _FPT(torch.autograd.Function):
         n = proj3.shape[0]
         out = torch.empty((n, p.ny, p.nx), dtype=proj3.dtype, device=proj3.device)
         w, pos, usex = t_grad
-        _lib.call("ctr_fp_t", C.byref(p.ctr()), _dtype_code(proj3), _ptr(proj3), _ptr(out), n,
-                  _ptr(w), _ptr(pos), _ptr(usex), max(p.nx, p.ny), _stream(proj3))
+        _call_batched("ctr_fp_t", n, C.byref(p.ctr()), _dtype_code(proj3), _ptr(proj3),
+                      _ptr(out), n, _ptr(w), _ptr(pos), _ptr(usex), max(p.nx, p.ny),
+                      _stream(proj3))
         return out
 
     @staticmethod
@@ -376,12 +386,13 @@ def _ramp_launch(x2, p, taps, wp, impl=None):
     wtype = 0
     if wp is not None:
         g.dso, g.dsd, g.offset_s, wtype = wp.dso, wp.dsd, wp.offset, wp.type
+    rows = x2.shape[0]
     if x2.dtype == torch.float32 and impl != "direct":
-        _lib.call("ctr_ramp_toeplitz", C.byref(g), _lib.CTR_F32, _ptr(x2), _ptr(out),
-                  x2.shape[0], _ptr(taps[0]), wtype, int(impl == "bf16x3"), _stream(x2))
+        _call_batched("ctr_ramp_toeplitz", rows, C.byref(g), _lib.CTR_F32, _ptr(x2), _ptr(out),
+                      rows, _ptr(taps[0]), wtype, int(impl == "bf16x3"), _stream(x2))
     else:
-        _lib.call("ctr_ramp", C.byref(g), _dtype_code(x2), _ptr(x2), _ptr(out), x2.shape[0],
-                  _ptr(taps[0]), wtype, _stream(x2))
+        _call_batched("ctr_ramp", rows, C.byref(g), _dtype_code(x2), _ptr(x2), _ptr(out), rows,
+                      _ptr(taps[0]), wtype, _stream(x2))
     return out
 
 
@@ -423,8 +434,8 @@ class _FanW(torch.autograd.Function):
     def forward(ctx, x2, p):
         ctx.p = p
         out = torch.empty_like(x2)
-        _lib.call("ctr_fan_weighting", C.byref(p.ctr()), _dtype_code(x2), _ptr(x2), _ptr(out),
-                  x2.shape[0], _stream(x2))
+        _call_batched("ctr_fan_weighting", x2.shape[0], C.byref(p.ctr()), _dtype_code(x2),
+                      _ptr(x2), _ptr(out), x2.shape[0], _stream(x2))
         return out
 
     @staticmethod
