@@ -82,6 +82,70 @@ __device__ __forceinline__ void stage_spans(const BpLaunch<Geo>& L,
     }
 }
 
+// Slice-interleaved staging, used when NB is a multiple of 4: a view's rows
+// are stored as [NB/4 groups][LROW channels][4 slices], so one 16-byte LDS
+// read returns a tap for four slices (gfx950 serves ds_read_b128 at ~1.6x
+// the bytes per clock of ds_read2st64_b32).  The DMA still writes LDS
+// lane-linearly (lane i -> word i of a 64-word block), so lane i fetches
+// slice i & 3 of channel i >> 2: per-lane global offsets from a descriptor
+// based at the group's first slice.  Zero fill: a slice past the batch has
+// an offset beyond the group's range; a channel off the detector gets an
+// offset of 2^31, beyond every range (the host keeps 4 slices < 2^31 bytes).
+template <class Geo, int NB, int LROW, int NWAVES>
+__device__ __forceinline__ void stage_spans_quad(const BpLaunch<Geo>& L,
+                                                 const typename Geo::Anchor* anchors, float* buf,
+                                                 int v, int vrel, int nv, int b0, int nb,
+                                                 int wave, int lane)
+{
+    static_assert(NB % 4 == 0 && LROW % 16 == 0, "four slices x 16 channels per DMA row");
+    constexpr int NC = LROW / 16;
+    const auto& P = L.P;
+    const unsigned int slice_bytes = static_cast<unsigned int>(P.na) * P.ns * 4u;
+    const size_t group_stride = 4 * static_cast<size_t>(P.na) * P.ns;
+    const unsigned int lane_off = (lane & 3) * slice_bytes;
+    const int lane_ch = lane >> 2;
+    for (int t = wave; t < nv; t += NWAVES) {
+        const int is0 = uniform_i(anchors[vrel + t].is0);
+        const bool span_in = is0 >= 0 && is0 + LROW <= P.ns;
+        unsigned int voff[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int ch = is0 + 16 * c + lane_ch;
+            voff[c] = (span_in || static_cast<unsigned int>(ch) < static_cast<unsigned int>(P.ns))
+                          ? lane_off + static_cast<unsigned int>(ch) * 4u
+                          : 0x80000000u;
+        }
+        const float* src = L.proj + (static_cast<size_t>(b0) * P.na + v + t) * P.ns;
+        float* dst = buf + t * NB * LROW;
+#pragma unroll
+        for (int g = 0; g < NB / 4; ++g) {
+            const int ng = nb - 4 * g;
+            const unsigned int range = ng <= 0 ? 0u : ng >= 4 ? 4u * slice_bytes : ng * slice_bytes;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<float*>(src), 0, range, 0x00020000);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4, voff[c], 0,
+                    0, 0);
+            src += group_stride;
+            dst += 4 * LROW;
+        }
+    }
+}
+
+template <class Geo, int NB, int LROW, int NWAVES>
+__device__ __forceinline__ void stage_views(const BpLaunch<Geo>& L,
+                                            const typename Geo::Anchor* anchors, float* buf,
+                                            int v, int vrel, int nv, int b0, int nb, int wave,
+                                            int lane)
+{
+    if constexpr (NB % 4 == 0)
+        stage_spans_quad<Geo, NB, LROW, NWAVES>(L, anchors, buf, v, vrel, nv, b0, nb, wave, lane);
+    else
+        stage_spans<Geo, NB, LROW, NWAVES>(L, anchors, buf, v, vrel, nv, b0, nb, wave, lane);
+}
+
 // Geometry of one pixel-view in the form the accumulation needs: LDS word
 // index of the left tap and the two tap weights.  Interpolating policies
 // (the default) give a channel position and a weight and get the linear
@@ -115,6 +179,95 @@ __device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
         idx = min(max(idx, 0), LROW - 2);
         w1 = w * fr;
         w0 = w - w1;
+    }
+}
+
+// One chunk of views of a tile: issue the DMA of the next chunk into `next`,
+// then accumulate the views of this chunk (staged in `sbuf`) into acc.  The
+// three LDS regions are disjoint, and the restrict qualifiers say so: the
+// compiler then knows the span reads cannot depend on the DMA still in
+// flight into `next` and inserts no wait for it.
+template <class Geo, int NB, int LROW, int NWAVES, int PPT>
+__device__ __forceinline__ void bp_chunk(const BpLaunch<Geo>& L,
+                                         const typename Geo::Anchor* __restrict__ anchors,
+                                         float* __restrict__ next, const float* __restrict__ sbuf,
+                                         bool prefetch, int va, int vc, int nv, int nv_next,
+                                         int b0, int nb, int wave, int lane,
+                                         const float (&fi)[PPT], const float (&fj)[PPT],
+                                         float (&acc)[PPT][NB])
+{
+    using Anchor = typename Geo::Anchor;
+    const auto& P = L.P;
+    if (prefetch)
+        stage_views<Geo, NB, LROW, NWAVES>(L, anchors, next, va + vc + L.vchunk, vc + L.vchunk,
+                                           nv_next, b0, nb, wave, lane);
+    // Software pipeline over the views of the chunk: the taps of view
+    // t are read while the geometry of view t+1 is computed.
+    int idx[PPT];
+    float w0[PPT], w1[PPT];
+    {
+        const Anchor A = anchors[vc];
+        const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+            tap_weights<Geo, LROW>(P, A, inside, is0, fi[k], fj[k], idx[k], w0[k], w1[k]);
+    }
+    for (int t = 0; t < nv; ++t) {
+        const float* sview = sbuf + t * NB * LROW;
+        float p0[PPT][NB], p1[PPT][NB];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            if constexpr (NB % 4 == 0) {
+                // [group][channel][4 slices]: one ds_read_b128 per tap
+                // and group, all at immediate offsets from one address
+                const float4* q = reinterpret_cast<const float4*>(sview) + idx[k];
+#pragma unroll
+                for (int g = 0; g < NB / 4; ++g) {
+                    const float4 a = q[g * LROW], c = q[g * LROW + 1];
+                    p0[k][4 * g] = a.x;
+                    p0[k][4 * g + 1] = a.y;
+                    p0[k][4 * g + 2] = a.z;
+                    p0[k][4 * g + 3] = a.w;
+                    p1[k][4 * g] = c.x;
+                    p1[k][4 * g + 1] = c.y;
+                    p1[k][4 * g + 2] = c.z;
+                    p1[k][4 * g + 3] = c.w;
+                }
+            } else {
+                // Right taps come through a second base whose +1 the
+                // compiler cannot see, so the loads pair up per tap
+                // across slices (rows LROW words apart:
+                // ds_read2st64_b32, no per-slice address arithmetic).
+                int one = 1;
+                asm volatile("" : "+v"(one));
+                const float* q0 = sview + idx[k];
+                const float* q1 = sview + idx[k] + one;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    p0[k][b] = q0[b * LROW];
+                    p1[k][b] = q1[b * LROW];
+                }
+            }
+        }
+        float cw0[PPT], cw1[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            cw0[k] = w0[k];
+            cw1[k] = w1[k];
+        }
+        if (t + 1 < nv) {
+            const Anchor A = anchors[vc + t + 1];
+            const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
+#pragma unroll
+            for (int k = 0; k < PPT; ++k)
+                tap_weights<Geo, LROW>(P, A, inside, is0, fi[k], fj[k], idx[k], w0[k],
+                                       w1[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+                acc[k][b] = fmaf(cw1[k], p1[k][b], fmaf(cw0[k], p0[k][b], acc[k][b]));
     }
 }
 
@@ -158,7 +311,7 @@ __device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpL
         __syncthreads();  // previous anchors and spans are no longer read
         if (tid < nva) anchors[tid] = Geo::anchor(P, T, F, va + tid, ilo, ihi, jlo, jhi);
         __syncthreads();
-        stage_spans<Geo, NB, LROW, NWAVES>(L, anchors, spans, va, 0, min(L.vchunk, nva), b0, nb,
+        stage_views<Geo, NB, LROW, NWAVES>(L, anchors, spans, va, 0, min(L.vchunk, nva), b0, nb,
                                            wave, lane);
         float acc[PPT][NB];
 #pragma unroll
@@ -171,63 +324,11 @@ __device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpL
             const int nv = min(L.vchunk, nva - vc);
             __builtin_amdgcn_s_waitcnt(0);  // this wave's span loads have landed
             __syncthreads();                // ... and every other wave's
-            // prefetch the next chunk into the other buffer
-            if (vc + L.vchunk < nva)
-                stage_spans<Geo, NB, LROW, NWAVES>(L, anchors, spans + (cur ^ 1) * bufsz,
-                                                   va + vc + L.vchunk, vc + L.vchunk,
-                                                   min(L.vchunk, nva - vc - L.vchunk), b0, nb,
-                                                   wave, lane);
-            const float* sbuf = spans + cur * bufsz;
-            // Software pipeline over the views of the chunk: the taps of view
-            // t are read while the geometry of view t+1 is computed.
-            int idx[PPT];
-            float w0[PPT], w1[PPT];
-            {
-                const Anchor A = anchors[vc];
-                const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
-#pragma unroll
-                for (int k = 0; k < PPT; ++k)
-                    tap_weights<Geo, LROW>(P, A, inside, is0, fi[k], fj[k], idx[k], w0[k], w1[k]);
-            }
-            for (int t = 0; t < nv; ++t) {
-                const float* sview = sbuf + t * NB * LROW;
-                float p0[PPT][NB], p1[PPT][NB];
-#pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    // Right taps come through a second base whose +1 the
-                    // compiler cannot see, so the loads pair up per tap across
-                    // slices (rows LROW words apart: ds_read2st64_b32, no
-                    // per-slice address arithmetic).
-                    int one = 1;
-                    asm volatile("" : "+v"(one));
-                    const float* q0 = sview + idx[k];
-                    const float* q1 = sview + idx[k] + one;
-#pragma unroll
-                    for (int b = 0; b < NB; ++b) {
-                        p0[k][b] = q0[b * LROW];
-                        p1[k][b] = q1[b * LROW];
-                    }
-                }
-                float cw0[PPT], cw1[PPT];
-#pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    cw0[k] = w0[k];
-                    cw1[k] = w1[k];
-                }
-                if (t + 1 < nv) {
-                    const Anchor A = anchors[vc + t + 1];
-                    const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
-#pragma unroll
-                    for (int k = 0; k < PPT; ++k)
-                        tap_weights<Geo, LROW>(P, A, inside, is0, fi[k], fj[k], idx[k], w0[k],
-                                               w1[k]);
-                }
-#pragma unroll
-                for (int k = 0; k < PPT; ++k)
-#pragma unroll
-                    for (int b = 0; b < NB; ++b)
-                        acc[k][b] = fmaf(cw1[k], p1[k][b], fmaf(cw0[k], p0[k][b], acc[k][b]));
-            }
+            // prefetch the next chunk into the other buffer, accumulate this one
+            bp_chunk<Geo, NB, LROW, NWAVES, PPT>(
+                L, anchors, spans + (cur ^ 1) * bufsz, spans + cur * bufsz,
+                vc + L.vchunk < nva, va, vc, nv, min(L.vchunk, nva - vc - L.vchunk), b0, nb, wave,
+                lane, fi, fj, acc);
             cur ^= 1;
         }
 #pragma unroll

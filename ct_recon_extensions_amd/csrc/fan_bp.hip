@@ -239,7 +239,7 @@ int fan_bp_f32(const ctr_geom& g, const float* proj, float* img, int nbatch, con
         if (rc) return rc;
     }
     const FanTables T{xpos, ypos, sc};
-    const int nb = cfg.nb ? cfg.nb : auto_nb(nbatch);
+    const int nb = engine_nb(cfg.nb ? cfg.nb : auto_nb(nbatch), g.na, g.ns);
 #define CTR_FAN_NB(TW_, TH_)                                                                   \
     switch (nb) {                                                                              \
         case 1: return launch_fan_bp_nb<TW_, TH_, 1>(g, P, lmax, exact, nbatch, cfg.vchunk,     \

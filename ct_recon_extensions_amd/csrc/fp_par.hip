@@ -559,9 +559,10 @@ int fpt_typed(const ctr_geom& g, const T* proj, T* img, int nbatch, const double
             // interior on the engine, then the border pixels exactly
             const int s = static_cast<int>(stride);
             int rc;
-            if (nbatch >= 8) rc = launch_fpt_engine<8>(g, proj, img, nbatch, w, pos, usex, s, st);
-            else if (nbatch >= 4) rc = launch_fpt_engine<4>(g, proj, img, nbatch, w, pos, usex, s, st);
-            else if (nbatch >= 2) rc = launch_fpt_engine<2>(g, proj, img, nbatch, w, pos, usex, s, st);
+            const int nb = engine_nb(nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch, g.na, g.ns);
+            if (nb >= 8) rc = launch_fpt_engine<8>(g, proj, img, nbatch, w, pos, usex, s, st);
+            else if (nb >= 4) rc = launch_fpt_engine<4>(g, proj, img, nbatch, w, pos, usex, s, st);
+            else if (nb >= 2) rc = launch_fpt_engine<2>(g, proj, img, nbatch, w, pos, usex, s, st);
             else rc = launch_fpt_engine<1>(g, proj, img, nbatch, w, pos, usex, s, st);
             if (rc) return rc;
             FpProblem P{static_cast<int>(g.ns), static_cast<int>(g.na), static_cast<int>(g.nx),

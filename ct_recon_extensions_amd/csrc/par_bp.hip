@@ -211,7 +211,8 @@ int par_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch
     const ParTables T{xcos, ysin};
     L.proj = static_cast<const float*>(proj);
     L.img = static_cast<float*>(img);
-    const int nb = nbatch >= 16 ? 16 : nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
+    const int nb = engine_nb(
+        nbatch >= 16 ? 16 : nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1, g.na, g.ns);
     switch (nb) {
         case 1: return launch_par_bp<TW, TH, 1>(L, T, st);
         case 2: return launch_par_bp<TW, TH, 2>(L, T, st);

@@ -33,6 +33,14 @@ __device__ __forceinline__ int xcd_tile(int id, int ntiles)
     return x * q + min(x, r) + k;
 }
 
+// Slices per workgroup for the BP engine: slice-interleaved staging
+// (a multiple of 4 slices) addresses four slices of na x ns floats through
+// one 32-bit buffer range, so at 512 MiB per slice it gives way to 2.
+inline int engine_nb(int nb, long long na, long long ns)
+{
+    return (nb > 2 && na * ns * 4 >= (1LL << 29)) ? 2 : nb;
+}
+
 // Origin of a tile and the pixel (ic, jc) its anchor is evaluated at.
 struct TileFrame {
     int x0, y0, ic, jc;
