@@ -61,6 +61,7 @@ _SIGS = {
     "ctr_fp_grad_tables": (C.c_int, [_G, _P, _P, _P, C.c_uint]),
     "ctr_ramp_taps": (C.c_int, [_G, _P]),
     "ctr_fan_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, _P]),
+    "ctr_fan_bp_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "ctr_fan_bp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, C.c_int, _P]),
     "ctr_plain_fan_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, _P]),
     "ctr_par_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P]),

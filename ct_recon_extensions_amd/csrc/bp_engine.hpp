@@ -2,15 +2,17 @@
 // operators (and the ragged batch): forward kernel and view-parallel scatter
 // transpose, parameterised by a geometry policy Geo that supplies
 //   Geo::Problem, Geo::Tables, Geo::Anchor (16-byte multiple, field `is0`
-//   = first staged channel, `inside` = tile strictly inside the detector),
+//   = first staged channel, `nch` = channels from is0 the tile's taps can
+//   touch, `inside` = tile strictly inside the detector),
 //   Geo::anchor(P, T, frame, view, ilo, ihi, jlo, jhi)   FP64 -> FP32 anchor
 //   Geo::pixel(P, A, fi, fj, f, w)   channel offset f (from is0) and weight.
 //
 // Forward layout choices (gfx950):
 //  * one workgroup = one pixel tile x NB slices of a batch that share the
 //    geometry; the geometry of a pixel-view is evaluated once for all NB;
-//  * per chunk of views the detector span each tile touches is staged into
-//    LDS as spans[view][slice][LROW channels] straight from global memory
+//  * per chunk of views the detector span each tile touches (nch channels,
+//    LROW at most) is staged into LDS as spans[view][slice][LROW channels]
+//    (or slice-interleaved, below) straight from global memory
 //    with buffer loads that write LDS directly (no VGPRs, no per-row wait;
 //    the buffer range check returns 0 for channels off the detector, which
 //    is the zero padding the edge rule needs), double-buffered so the next
@@ -64,6 +66,8 @@ __device__ __forceinline__ void stage_spans(const BpLaunch<Geo>& L,
     const size_t slice_stride = static_cast<size_t>(P.na) * P.ns;
     for (int t = wave; t < nv; t += NWAVES) {
         const int is0 = uniform_i(anchors[vrel + t].is0);
+        // only the 64-channel rows this view's taps can reach
+        const int nrow = min(LROW / 64, (uniform_i(anchors[vrel + t].nch) + 63) / 64);
         const int voff = (is0 + lane) * 4;
         const float* src = L.proj + (static_cast<size_t>(b0) * P.na + v + t) * P.ns;
         float* dst = buf + t * NB * LROW;
@@ -71,11 +75,10 @@ __device__ __forceinline__ void stage_spans(const BpLaunch<Geo>& L,
         for (int b = 0; b < NB; ++b) {
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<float*>(src), 0, b < nb ? row_bytes : 0u, 0x00020000);
-#pragma unroll
-            for (int c = 0; c < LROW; c += 64)
+            for (int c = 0; c < nrow; ++c)
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    rs, (__attribute__((address_space(3))) void*)(dst + c), 4, voff + c * 4, 0,
-                    0, 0);
+                    rs, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4,
+                    voff + c * 256, 0, 0, 0);
             src += slice_stride;
             dst += LROW;
         }
@@ -106,7 +109,10 @@ __device__ __forceinline__ void stage_spans_quad(const BpLaunch<Geo>& L,
     const int lane_ch = lane >> 2;
     for (int t = wave; t < nv; t += NWAVES) {
         const int is0 = uniform_i(anchors[vrel + t].is0);
-        const bool span_in = is0 >= 0 && is0 + LROW <= P.ns;
+        // only the 16-channel rows this view's taps can reach (about half of
+        // LROW on typical fan and parallel geometries)
+        const int nrow = min(NC, (uniform_i(anchors[vrel + t].nch) + 15) / 16);
+        const bool span_in = is0 >= 0 && is0 + 16 * nrow <= P.ns;
         unsigned int voff[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -125,9 +131,10 @@ __device__ __forceinline__ void stage_spans_quad(const BpLaunch<Geo>& L,
                 const_cast<float*>(src), 0, range, 0x00020000);
 #pragma unroll
             for (int c = 0; c < NC; ++c)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    rs, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4, voff[c], 0,
-                    0, 0);
+                if (c < nrow)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rs, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4, voff[c],
+                        0, 0, 0);
             src += group_stride;
             dst += 4 * LROW;
         }

@@ -28,6 +28,7 @@ int fp_par(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch
 int fp_par_t(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
              const double* w, const double* pos, const int* usex, unsigned int pos_stride,
              hipStream_t st);
+void fan_bp_set_config(int tile, int vchunk, int nb);
 int ramp(const ctr_geom& g, int dtype, const void* in, void* out, unsigned int nrow,
          const double* filter, int weight_type, hipStream_t st);
 int fan_weighting(const ctr_geom& g, int dtype, const void* in, void* out, unsigned int nrow,
@@ -87,6 +88,17 @@ int ctr_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nb
         return ctr::fan_bp_f64(*g, static_cast<const double*>(proj), static_cast<double*>(img),
                                nbatch, xpos, ypos, sincostbl, scale, st);
     });
+}
+
+int ctr_fan_bp_config(int tile, int vchunk, int nb)
+{
+    CTR_REQUIRE(tile >= 0 && tile <= 2, CTR_ERR_ARG, "ctr_fan_bp_config: tile must be 0, 1 or 2");
+    CTR_REQUIRE(vchunk >= 1 && vchunk <= 64, CTR_ERR_ARG,
+                "ctr_fan_bp_config: vchunk must be in [1, 64]");
+    CTR_REQUIRE(nb == 0 || nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16, CTR_ERR_ARG,
+                "ctr_fan_bp_config: nb must be 0, 1, 2, 4, 8 or 16");
+    ctr::fan_bp_set_config(tile, vchunk, nb);
+    return CTR_OK;
 }
 
 int ctr_fan_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,

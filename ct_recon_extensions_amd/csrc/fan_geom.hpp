@@ -33,6 +33,8 @@ struct alignas(16) FanAnchor {
     float fc;      // anchor channel position minus is0
     int is0;       // first channel of the staged detector span
     int inside;    // 1: every pixel of the tile lands strictly inside [0, ns-1]
+    int nch;       // channels from is0 the tile's taps can touch (one of margin)
+    int pad[3];
 };
 
 // Geometry of one fan problem as the kernels see it.
@@ -97,6 +99,8 @@ __device__ inline FanAnchor fan_anchor(const FanProblem& P, double xc, double yc
     A.fc = static_cast<float>(sc - is0);
     A.is0 = is0;
     A.inside = (smin >= 0.25 && smax <= P.ns - 1.25) ? 1 : 0;
+    A.nch = static_cast<int>(floor(smax)) + 3 - is0;
+    A.pad[0] = A.pad[1] = A.pad[2] = 0;
     return A;
 }
 

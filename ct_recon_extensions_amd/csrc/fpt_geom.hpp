@@ -25,7 +25,8 @@ struct alignas(16) FptAnchor {
     float wv;      // the view's weight |dy/cos| or |dx/sin|
     int is0;
     int inside;    // unused by the custom taps (kept for the engine)
-    int pad;
+    int nch;       // channels from is0 the tile's taps can touch (one of margin)
+    int pad[3];
 };
 
 struct FptProblem {
@@ -66,6 +67,7 @@ struct FptGeo {
             b = o1;
         }
         const double lo = fmin(a * ilo, a * ihi) + fmin(b * jlo, b * jhi);
+        const double hi = fmax(a * ilo, a * ihi) + fmax(b * jlo, b * jhi);
         const int is0 = static_cast<int>(floor(midc + lo)) - 1;
         A.a = static_cast<float>(a);
         A.b = static_cast<float>(b);
@@ -74,7 +76,8 @@ struct FptGeo {
         A.wv = static_cast<float>(T.weights[view]);
         A.is0 = is0;
         A.inside = 1;
-        A.pad = 0;
+        A.nch = static_cast<int>(floor(midc + hi)) + 3 - is0;
+        A.pad[0] = A.pad[1] = A.pad[2] = 0;
         return A;
     }
 

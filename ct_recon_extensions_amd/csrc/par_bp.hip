@@ -22,7 +22,8 @@ struct alignas(16) ParAnchor {
     float fc;    // anchor channel minus is0
     int is0;     // first staged channel
     int inside;  // whole tile strictly inside the detector
-    int pad[3];
+    int nch;     // channels from is0 the tile's taps can touch (one of margin)
+    int pad[2];
 };
 
 struct ParProblem {
@@ -50,7 +51,8 @@ __device__ inline ParAnchor par_anchor(const ParProblem& P, const double* __rest
     A.fc = static_cast<float>(sc - is0);
     A.is0 = is0;
     A.inside = (sc + lo >= 0.25 && sc + hi <= P.ns - 1.25) ? 1 : 0;
-    A.pad[0] = A.pad[1] = A.pad[2] = 0;
+    A.nch = static_cast<int>(floor(sc + hi)) + 3 - is0;
+    A.pad[0] = A.pad[1] = 0;
     return A;
 }
 

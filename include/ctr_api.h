@@ -102,6 +102,14 @@ int ctr_ramp_taps(const ctr_geom* g, double* filter);
 int ctr_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
                const double* xpos, const double* ypos, const double* sincostbl, double scale,
                void* stream);
+/* Launch configuration of ctr_fan_bp's float32 kernel for the calling
+ * process: tile 0 = 16x16, 1 = 8x8, 2 = 32x8 pixels; vchunk = views per LDS
+ * round (1..64, capped to the LDS budget); nb = slices per workgroup
+ * (1, 2, 4, 8, 16; 0 = from the batch).  Defaults 0, 4, 0.  Tiles whose
+ * detector span exceeds 256 channels switch to 8x8 by themselves.  No
+ * counterpart in the reference, whose launch shape is fixed
+ * (cuda/bp_fan_2d.cu:219-245). */
+int ctr_fan_bp_config(int tile, int vchunk, int nb);
 /* Exact transpose of ctr_fan_bp: img [nbatch][ny][nx] -> proj
  * [nbatch][na][ns] (replaces FanBackprojection2DPixDrivenGrad<T>,
  * src/bp_fan_2d.cc:124-170, and the broken bp_fan_2d_angle transpose).
