@@ -221,6 +221,11 @@ __device__ __forceinline__ void bp_chunk(const BpLaunch<Geo>& L,
     }
     for (int t = 0; t < nv; ++t) {
         const float* sview = sbuf + t * NB * LROW;
+        // LDS returns in order: the next view's anchor is read before this
+        // view's taps (the empty asm keeps the compiler from sinking the read
+        // below them), so the next geometry need not wait for the taps.
+        const Anchor An = anchors[vc + min(t + 1, nv - 1)];
+        asm volatile("" ::: "memory");
         float p0[PPT][NB], p1[PPT][NB];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
@@ -263,11 +268,10 @@ __device__ __forceinline__ void bp_chunk(const BpLaunch<Geo>& L,
             cw1[k] = w1[k];
         }
         if (t + 1 < nv) {
-            const Anchor A = anchors[vc + t + 1];
-            const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
+            const int inside = uniform_i(An.inside), is0 = uniform_i(An.is0);
 #pragma unroll
             for (int k = 0; k < PPT; ++k)
-                tap_weights<Geo, LROW>(P, A, inside, is0, fi[k], fj[k], idx[k], w0[k],
+                tap_weights<Geo, LROW>(P, An, inside, is0, fi[k], fj[k], idx[k], w0[k],
                                        w1[k]);
         }
 #pragma unroll

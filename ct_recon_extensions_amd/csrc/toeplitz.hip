@@ -33,6 +33,7 @@
 // K-chunks of 64 (64 x 64 words, padded); the tap vector covers the whole
 // row.
 #include "ctr_common.hpp"
+#include "tile_common.hpp"
 
 #include <hip/hip_bf16.h>
 
@@ -60,8 +61,22 @@ struct ToeplitzParams {
     int ns, nrow;
     int n_even, n_odd;   // outputs of each parity
     int tiles_even;      // column tiles of parity 0 (the rest are parity 1)
+    int col_tiles;       // column tiles of both parities
+    int row_tiles;
     double ds;
 };
+
+// Workgroup -> (row tile bx, column tile by).  The grid is one-dimensional
+// and workgroups go to the 8 XCDs round-robin; xcd_tile hands every XCD a
+// contiguous run of logical indices in which the column tiles of one row
+// block are consecutive, so the block's input rows come from HBM once and
+// are re-read from that XCD's L2 by its other column tiles.
+__device__ __forceinline__ void toeplitz_tile(const ToeplitzParams& P, int& bx, int& by)
+{
+    const int l = xcd_tile(blockIdx.x, P.row_tiles * P.col_tiles);
+    bx = l / P.col_tiles;
+    by = l - bx * P.col_tiles;
+}
 
 __device__ __forceinline__ float wt(const ToeplitzParams& P, int k)
 {
@@ -143,9 +158,11 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
     float* a_lds = tap + ((nq + 3) & ~3);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int r0 = blockIdx.x * kTM;
-    const int pi = blockIdx.y < P.tiles_even ? 0 : 1;
-    const int j0 = (pi == 0 ? blockIdx.y : blockIdx.y - P.tiles_even) * kTN;
+    int bx, by;
+    toeplitz_tile(P, bx, by);
+    const int r0 = bx * kTM;
+    const int pi = by < P.tiles_even ? 0 : 1;
+    const int j0 = (pi == 0 ? by : by - P.tiles_even) * kTN;
     const int nout = pi == 0 ? P.n_even : P.n_odd;
     const int kdim = pi == 0 ? P.ns / 2 : (P.ns + 1) / 2;  // inputs of parity 1-pi
     const int kpar = 1 - pi;
@@ -284,9 +301,11 @@ __global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P
     __bf16* alo = ahi + kTM * kAS;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int r0 = blockIdx.x * kTM;
-    const int pi = blockIdx.y < P.tiles_even ? 0 : 1;
-    const int j0 = (pi == 0 ? blockIdx.y : blockIdx.y - P.tiles_even) * kTN;
+    int bx, by;
+    toeplitz_tile(P, bx, by);
+    const int r0 = bx * kTM;
+    const int pi = by < P.tiles_even ? 0 : 1;
+    const int j0 = (pi == 0 ? by : by - P.tiles_even) * kTN;
     const int nout = pi == 0 ? P.n_even : P.n_odd;
     const int kdim = pi == 0 ? P.ns / 2 : (P.ns + 1) / 2;
     const int kpar = 1 - pi;
@@ -472,7 +491,9 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
             CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         static_cast<int>(lds)));
-        const dim3 grid(div_up(nrow, kTM), P.tiles_even + div_up(P.n_odd, kTN));
+        P.col_tiles = P.tiles_even + div_up(P.n_odd, kTN);
+        P.row_tiles = static_cast<int>(div_up(nrow, kTM));
+        const dim3 grid(static_cast<unsigned int>(P.row_tiles) * P.col_tiles);
         hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, P);
         rc = check_launch("ramp_mfma_kernel");
     }
