@@ -63,6 +63,7 @@ struct ToeplitzParams {
     int tiles_even;      // column tiles of parity 0 (the rest are parity 1)
     int col_tiles;       // column tiles of both parities
     int row_tiles;
+    int grid_cols;       // workgroups per row block (tiles, or groups of tiles)
     double ds;
 };
 
@@ -73,9 +74,9 @@ struct ToeplitzParams {
 // are re-read from that XCD's L2 by its other column tiles.
 __device__ __forceinline__ void toeplitz_tile(const ToeplitzParams& P, int& bx, int& by)
 {
-    const int l = xcd_tile(blockIdx.x, P.row_tiles * P.col_tiles);
-    bx = l / P.col_tiles;
-    by = l - bx * P.col_tiles;
+    const int l = xcd_tile(blockIdx.x, P.row_tiles * P.grid_cols);
+    bx = l / P.grid_cols;
+    by = l - bx * P.grid_cols;
 }
 
 __device__ __forceinline__ float wt(const ToeplitzParams& P, int k)
@@ -281,8 +282,9 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 __host__ __device__ __forceinline__ int rev_len(int nq) { return ((nq + 7) & ~7) + 8; }
 
 // NC tap copies: 8 (each fragment one ds_read_b128) or 4 (two ds_read_b64,
-// half the LDS).
-template <int NC>
+// half the LDS).  NT: column tiles of one parity per workgroup, all fed by
+// the same staged A chunks (the row block's input is read once per NT tiles).
+template <int NC, int NT>
 __global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -304,8 +306,12 @@ __global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P
     int bx, by;
     toeplitz_tile(P, bx, by);
     const int r0 = bx * kTM;
-    const int pi = by < P.tiles_even ? 0 : 1;
-    const int j0 = (pi == 0 ? by : by - P.tiles_even) * kTN;
+    // column groups of NT tiles: ceil(tiles_even / NT) of parity 0, then parity 1
+    const int groups_even = (P.tiles_even + NT - 1) / NT;
+    const int pi = by < groups_even ? 0 : 1;
+    const int ntiles_pi = pi == 0 ? P.tiles_even : P.col_tiles - P.tiles_even;
+    const int t0 = (pi == 0 ? by : by - groups_even) * NT;  // first tile of the group
+    const int ntv = min(NT, ntiles_pi - t0);                  // its valid tiles
     const int nout = pi == 0 ? P.n_even : P.n_odd;
     const int kdim = pi == 0 ? P.ns / 2 : (P.ns + 1) / 2;
     const int kpar = 1 - pi;
@@ -334,11 +340,13 @@ __global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P
     load_chunk(0);
 
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    f32x4 acc[2][2];
+    f32x4 acc[NT][2][2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[tt][a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
     for (int i0 = 0; i0 < kdim; i0 += kKC) {
         __syncthreads();  // B copies / weights written, previous chunk consumed
@@ -353,47 +361,58 @@ __global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P
         }
         __syncthreads();
         if (i0 + kKC < kdim) load_chunk(i0 + kKC);  // in flight during the MFMAs
-        const int tapbase = j0 + wc - i0 + pi - 1 - qmin;  // G[i][j] = ho[j - i + pi - 1]
 #pragma unroll
         for (int ik = 0; ik < kKC; ik += 32) {
             // lane: A[m = lane%16][k = 8(lane/16) + e], B[k = 8(lane/16) + e][n = lane%16]
             const int m = lane & 15, kb = ik + 8 * (lane >> 4);
-            bf16x8 fa_hi[2], fa_lo[2], fb_hi[2], fb_lo[2];
+            bf16x8 fa_hi[2], fa_lo[2];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 const int ro = (wr + 16 * t + m) * kAS + kb;
                 fa_hi[t] = *reinterpret_cast<const bf16x8*>(ahi + ro);
                 fa_lo[t] = *reinterpret_cast<const bf16x8*>(alo + ro);
-                // taps Q, Q-1, ..., Q-7 = reversed entries S .. S+7, S = nq-1-Q
-                const int S = nq - 1 - (tapbase + 16 * t + m - kb);
-                const int c = S & (NC - 1), bo = c * nrev + (S - c);
-                if constexpr (NC == 8) {
-                    fb_hi[t] = *reinterpret_cast<const bf16x8*>(bhi + bo);
-                    fb_lo[t] = *reinterpret_cast<const bf16x8*>(blo + bo);
-                } else {
-                    const bf16x4 h0 = *reinterpret_cast<const bf16x4*>(bhi + bo);
-                    const bf16x4 h1 = *reinterpret_cast<const bf16x4*>(bhi + bo + 4);
-                    const bf16x4 l0 = *reinterpret_cast<const bf16x4*>(blo + bo);
-                    const bf16x4 l1 = *reinterpret_cast<const bf16x4*>(blo + bo + 4);
-                    fb_hi[t] = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
-                    fb_lo[t] = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
-                }
             }
 #pragma unroll
-            for (int ta = 0; ta < 2; ++ta)
+            for (int tt = 0; tt < NT; ++tt) {
+                if (tt >= ntv) break;  // uniform: past the parity's last tile
+                // G[i][j] = ho[j - i + pi - 1]
+                const int tapbase = (t0 + tt) * kTN + wc - i0 + pi - 1 - qmin;
+                bf16x8 fb_hi[2], fb_lo[2];
 #pragma unroll
-                for (int tb = 0; tb < 2; ++tb) {
-                    acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_lo[ta], fb_hi[tb],
-                                                                          acc[ta][tb], 0, 0, 0);
-                    acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi[ta], fb_lo[tb],
-                                                                          acc[ta][tb], 0, 0, 0);
-                    acc[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi[ta], fb_hi[tb],
-                                                                          acc[ta][tb], 0, 0, 0);
+                for (int t = 0; t < 2; ++t) {
+                    // taps Q, Q-1, ..., Q-7 = reversed entries S .. S+7, S = nq-1-Q
+                    const int S = nq - 1 - (tapbase + 16 * t + m - kb);
+                    const int c = S & (NC - 1), bo = c * nrev + (S - c);
+                    if constexpr (NC == 8) {
+                        fb_hi[t] = *reinterpret_cast<const bf16x8*>(bhi + bo);
+                        fb_lo[t] = *reinterpret_cast<const bf16x8*>(blo + bo);
+                    } else {
+                        const bf16x4 h0 = *reinterpret_cast<const bf16x4*>(bhi + bo);
+                        const bf16x4 h1 = *reinterpret_cast<const bf16x4*>(bhi + bo + 4);
+                        const bf16x4 l0 = *reinterpret_cast<const bf16x4*>(blo + bo);
+                        const bf16x4 l1 = *reinterpret_cast<const bf16x4*>(blo + bo + 4);
+                        fb_hi[t] = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
+                        fb_lo[t] = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
+                    }
                 }
+#pragma unroll
+                for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+                    for (int tb = 0; tb < 2; ++tb) {
+                        f32x4& d = acc[tt][ta][tb];
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_lo[ta], fb_hi[tb], d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi[ta], fb_lo[tb], d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi[ta], fb_hi[tb], d, 0, 0, 0);
+                    }
+            }
         }
     }
-    near_field_epilogue(P, wl, reinterpret_cast<float*>(bbase), acc, r0, j0, pi, kpar, kdim, nout,
-                        wr, wc, tid, lane);
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt) {
+        if (tt >= ntv) break;
+        near_field_epilogue(P, wl, reinterpret_cast<float*>(bbase), acc[tt], r0, (t0 + tt) * kTN,
+                            pi, kpar, kdim, nout, wr, wc, tid, lane);
+    }
 }
 
 // The BF16x3 kernel's tap operand: copy c (c < nc), entry j holds the
@@ -435,6 +454,9 @@ __global__ void toeplitz_prep_kernel(const double* __restrict__ h, float* __rest
 // LDS at 1472 channels) measured 0.33 ms on 8 x 1440 rows of 1472 against
 // 0.42 ms for 8 copies (one ds_read_b128, 76 KB: two workgroups per CU).
 constexpr int kBf16Copies = 4;
+// Column tiles per BF16x3 workgroup: 2 measured fastest on 736- and
+// 1472-channel rows (3 and 6 lose occupancy to the extra accumulators).
+constexpr int kBf16Tiles = 2;
 static int bf16_copies() { return kBf16Copies; }
 
 int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int nrow,
@@ -486,14 +508,18 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
                   static_cast<size_t>(2) * kTM * kAS * 2;
         }
         CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "Toeplitz ramp filter: ns too large");
-        auto kern = bf16x3 ? ramp_bf16x3_kernel<kBf16Copies> : ramp_mfma_kernel<false>;
+        auto kern = bf16x3 ? ramp_bf16x3_kernel<kBf16Copies, kBf16Tiles> : ramp_mfma_kernel<false>;
         if (lds > 64 * 1024)
             CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         static_cast<int>(lds)));
         P.col_tiles = P.tiles_even + div_up(P.n_odd, kTN);
         P.row_tiles = static_cast<int>(div_up(nrow, kTM));
-        const dim3 grid(static_cast<unsigned int>(P.row_tiles) * P.col_tiles);
+        // BF16x3: one workgroup per group of kBf16Tiles column tiles of a parity
+        P.grid_cols = bf16x3 ? div_up(P.tiles_even, kBf16Tiles) +
+                                   div_up(P.col_tiles - P.tiles_even, kBf16Tiles)
+                             : P.col_tiles;
+        const dim3 grid(static_cast<unsigned int>(P.row_tiles) * P.grid_cols);
         hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, P);
         rc = check_launch("ramp_mfma_kernel");
     }
