@@ -64,8 +64,33 @@ struct ToeplitzParams {
     int col_tiles;       // column tiles of both parities
     int row_tiles;
     int grid_cols;       // workgroups per row block (tiles, or groups of tiles)
-    double ds;
+    int wtype;           // fused weighting: 0 none, 1 fan, 2 flat
+    double ds, dso, dsd, cents;
 };
+
+// Per-channel cosine weight dso |cos(gamma_k)| / dsd of FanWeighting<T>
+// (src/fan_weighting.cc:19-35): gamma = s / dsd (fan) or atan2(s, dsd)
+// (flat), s = ds (k - cents).
+__device__ __forceinline__ float cos_weight(int k, int wtype, double ds, double dso, double dsd,
+                                            double cents)
+{
+    const double s = ds * (static_cast<double>(k) - cents);
+    const double c = wtype == 2 ? cos(atan2(s, dsd)) : cos(s / dsd);
+    return static_cast<float>(dso * fabs(c) / dsd);
+}
+
+// Entry jc of the reversed far-field tap vector: ho[q] = h[2q+1] at
+// q = nq - 1 - jc + qmin, zero for the near taps (|q + 1/2| < kNear, carried
+// in FP64 by the epilogues) and past the filter's ends.
+__device__ __forceinline__ float far_tap_entry(const double* __restrict__ h, int ns, int nq,
+                                               int qmin, int jc)
+{
+    const int tj = nq - 1 - jc;
+    if (tj < 0) return 0.0f;
+    const int q = tj + qmin, m = 2 * q + 1;
+    const bool near = q >= -kNear && q < kNear;
+    return (m >= -ns && m <= ns && !near) ? static_cast<float>(h[ns + m]) : 0.0f;
+}
 
 // Workgroup -> (row tile bx, column tile by).  The grid is one-dimensional
 // and workgroups go to the 8 XCDs round-robin; xcd_tile hands every XCD a
@@ -415,22 +440,193 @@ __global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P
     }
 }
 
+// Whole-row BF16x3 filter for rows of up to ~900 channels: one workgroup
+// filters kRows rows.  The rows are loaded once, fan-weighted, into LDS in
+// FP32 with kRowPad zeros on the left and at least 64 on the right:
+//   xr[r][kRowPad + k] = w[k] x[r0 + r][k].
+// Wave w computes output parity pi = w >> 1 and half w & 1 of that parity's
+// 16-wide column tiles (TPW of them, the half rounded up to an even count):
+// the far field on the matrix cores (A fragments split
+// into BF16 hi / lo from the FP32 rows as they are read, B fragments from
+// the reversed tap copies, as in ramp_bf16x3_kernel), then the centre and
+// near taps in FP64 from the same rows.  The results go back into the row
+// buffer and leave as whole rows, so input and output move through HBM once
+// each, with contiguous accesses.
+constexpr int kRows = 16, kRowPad = 16;
+
+// Row stride (floats): kRowPad + ns + >= 66 zeros, odd, so that the 64
+// lanes of an A-fragment read (16 rows x 4 column groups 16 floats apart)
+// hit 64 different banks.
+__host__ __device__ __forceinline__ int row_stride(int ns) { return (kRowPad + ns + 66) | 1; }
+
+template <int NC, int TPW>
+__global__ __launch_bounds__(256) void ramp_rows_bf16x3_kernel(const ToeplitzParams P)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ns = P.ns;
+    const int qmin = -(ns / 2) - kKC - 4;
+    const int nq = ns + kKC + kTN + 10;
+    const int nrev = rev_len(nq);
+    const int xs = row_stride(ns);
+    // LDS: B copies hi/lo [NC][nrev] bf16 | rows [kRows][xs] floats
+    __bf16* bhi = reinterpret_cast<__bf16*>(smem);
+    __bf16* blo = bhi + NC * nrev;
+    float* xr = reinterpret_cast<float*>(smem + ((static_cast<size_t>(4 * NC) * nrev + 15) &
+                                                 ~static_cast<size_t>(15)));
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r0 = blockIdx.x * kRows;
+
+    // the tap copies (the layout bf16_tap_copies_kernel writes for the tiled
+    // kernel), straight from the FP64 taps
+    for (int e = tid; e < NC * nrev; e += 256) {
+        const int c = e / nrev, j = e - c * nrev;
+        const float f = far_tap_entry(P.taps64, ns, nq, qmin, j + c);
+        const __bf16 h = static_cast<__bf16>(f);
+        bhi[e] = h;
+        blo[e] = static_cast<__bf16>(f - static_cast<float>(h));
+    }
+    // Rows in, weighted: channel k = tid + 256 c (ns <= 1024), eight rows'
+    // loads in flight at a time; zeros in the margins.
+    float wk[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int k = tid + 256 * c;
+        wk[c] = k >= ns ? 0.0f
+                : P.wtype ? cos_weight(k, P.wtype, P.ds, P.dso, P.dsd, P.cents) : 1.0f;
+    }
+#pragma unroll
+    for (int rb = 0; rb < kRows; rb += 8) {
+        float v[8][4];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int k = tid + 256 * c, row = r0 + rb + r;
+                v[r][c] = (k < ns && row < P.nrow) ? P.x[static_cast<size_t>(row) * ns + k] : 0.0f;
+            }
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int k = tid + 256 * c;
+                if (k < ns) xr[(rb + r) * xs + kRowPad + k] = v[r][c] * wk[c];
+            }
+    }
+    const int npad = xs - ns;  // kRowPad on the left, the rest on the right
+    for (int e = tid; e < kRows * npad; e += 256) {
+        const int r = e / npad, c = e - r * npad;
+        xr[r * xs + (c < kRowPad ? c : ns + c)] = 0.0f;
+    }
+    __syncthreads();
+
+    const int pi = wave >> 1, half = wave & 1, kpar = 1 - pi;
+    const int nout = pi == 0 ? P.n_even : P.n_odd;
+    const int kdim = pi == 0 ? ns / 2 : (ns + 1) / 2;
+    const int nt = (nout + 15) / 16;
+    const int tper = (nt + 1) / 2;
+    const int tb0 = half * tper, ntw = min(tper, nt - tb0);  // this wave's tiles
+    const int m = lane & 15, g = lane >> 4;
+
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int i0 = 0; i0 < kdim; i0 += 32) {
+        // A[m][k]: input i = i0 + 8g + e of parity kpar in row m (zero past kdim)
+        const float* arow = xr + m * xs + kRowPad + 2 * (i0 + 8 * g) + kpar;
+        bf16x8 fa_hi, fa_lo;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float a = arow[2 * e];
+            const __bf16 h = static_cast<__bf16>(a);
+            fa_hi[e] = h;
+            fa_lo[e] = static_cast<__bf16>(a - static_cast<float>(h));
+        }
+        // every wave runs TPW tiles (at most one past the parity's last: its
+        // taps are still inside the copies, and its results are dropped)
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+            {
+            // B[k][n] = ho[q], q = j - i + pi - 1, j = 16 (tb0 + t) + n, i = i0 + 8g + e;
+            // taps Q, Q-1, ..., Q-7 = reversed entries S .. S+7, S = nq-1-Q
+            const int Q = 16 * (tb0 + t) + m - (i0 + 8 * g) + pi - 1 - qmin;
+            const int S = nq - 1 - Q;
+            const int c = S & (NC - 1), bo = c * nrev + (S - c);
+            bf16x8 fb_hi, fb_lo;
+            if constexpr (NC == 8) {
+                fb_hi = *reinterpret_cast<const bf16x8*>(bhi + bo);
+                fb_lo = *reinterpret_cast<const bf16x8*>(blo + bo);
+            } else {
+                const bf16x4 h0 = *reinterpret_cast<const bf16x4*>(bhi + bo);
+                const bf16x4 h1 = *reinterpret_cast<const bf16x4*>(bhi + bo + 4);
+                const bf16x4 l0 = *reinterpret_cast<const bf16x4*>(blo + bo);
+                const bf16x4 l1 = *reinterpret_cast<const bf16x4*>(blo + bo + 4);
+                fb_hi = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
+                fb_lo = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_lo, fb_hi, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi, fb_lo, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi, fb_hi, acc[t], 0, 0, 0);
+            }
+        }
+    }
+
+    // out[n] = ds (h0 xw[n] + sum_q h[2q+1] xw[n-1-2q] + far), n = 2j + pi,
+    // centre and |2q+1| <= 2 kNear - 1 in FP64; D[4g + u][m] is row 4g + u,
+    // column j = 16 (tb0 + t) + m.
+    const double h0 = P.taps64[ns];
+    double hn[2 * kNear];
+#pragma unroll
+    for (int q = -kNear; q < kNear; ++q) {
+        const int mm = 2 * q + 1;
+        hn[q + kNear] = (mm >= -ns && mm <= ns) ? P.taps64[ns + mm] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        if (t >= ntw) continue;
+        const int j = 16 * (tb0 + t) + m;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float* row = xr + (4 * g + u) * xs + kRowPad + 2 * j + pi;
+            double near = h0 * static_cast<double>(row[0]);
+#pragma unroll
+            for (int q = -kNear; q < kNear; ++q)
+                near = fma(hn[q + kNear], static_cast<double>(row[-1 - 2 * q]), near);
+            acc[t][u] = static_cast<float>(P.ds * (near + static_cast<double>(acc[t][u])));
+        }
+    }
+    __syncthreads();  // every wave has read the rows: they take the results
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int j = 16 * (tb0 + t) + m;
+        if (t < ntw && j < nout) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xr[(4 * g + u) * xs + kRowPad + 2 * j + pi] = acc[t][u];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        if (r0 + r >= P.nrow) break;  // uniform
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int k = tid + 256 * c;
+            if (k < ns) P.out[static_cast<size_t>(r0 + r) * ns + k] = xr[r * xs + kRowPad + k];
+        }
+    }
+}
+
 // The BF16x3 kernel's tap operand: copy c (c < nc), entry j holds the
 // reversed far-field tap nq-1-(j+c) split as hi (first nc nrev entries) and
 // lo (next nc nrev entries), i.e. exactly its LDS image.
-__global__ void bf16_tap_copies_kernel(const float* __restrict__ hf, __bf16* __restrict__ out,
+__global__ void bf16_tap_copies_kernel(const double* __restrict__ h64, __bf16* __restrict__ out,
                                        int ns, int nq, int nrev, int qmin, int nc)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nc * nrev) return;
     const int c = e / nrev, j = e - c * nrev;
-    const int tj = nq - 1 - (j + c);
-    float f = 0.0f;
-    if (tj >= 0) {
-        const int q = tj + qmin, m = 2 * q + 1;
-        const bool near = q >= -kNear && q < kNear;
-        f = (m >= -ns && m <= ns && !near) ? hf[ns + m] : 0.0f;
-    }
+    const float f = far_tap_entry(h64, ns, nq, qmin, j + c);
     const __bf16 h = static_cast<__bf16>(f);
     out[e] = h;
     out[nc * nrev + e] = static_cast<__bf16>(f - static_cast<float>(h));
@@ -443,11 +639,7 @@ __global__ void toeplitz_prep_kernel(const double* __restrict__ h, float* __rest
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 2 * ns + 1) hf[i] = static_cast<float>(h[i]);
-    if (w && i < ns) {
-        const double s = ds * (static_cast<double>(i) - cents);
-        const double c = wtype == 2 ? cos(atan2(s, dsd)) : cos(s / dsd);
-        w[i] = static_cast<float>(dso * fabs(c) / dsd);
-    }
+    if (w && i < ns) w[i] = cos_weight(i, wtype, ds, dso, dsd, cents);
 }
 
 // Tap copies of the BF16x3 kernel: 4 (two ds_read_b64 per fragment, 50 KB of
@@ -466,9 +658,47 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
     const int ns = static_cast<int>(g.ns);
     const int nq = ns + kKC + kTN + 10;
     const int nrev = ((nq + 7) & ~7) + 8;
+    const int nc = bf16_copies();
+    if (bf16x3) {
+        // Whole rows in LDS while two workgroups fit per CU and a wave's tiles
+        // fit its accumulators: the kernel builds its own tap copies and
+        // weights (no scratch, no preparation launches).
+        const size_t rows_lds = ((static_cast<size_t>(4 * nc) * nrev + 15) & ~static_cast<size_t>(15)) +
+                                static_cast<size_t>(kRows) * row_stride(ns) * sizeof(float);
+        const int n_even = (ns + 1) / 2;
+        const int tper = (div_up(n_even, 16) + 1) / 2;
+        if (rows_lds <= 80 * 1024 && tper <= 16 && ns <= 1024) {
+            ToeplitzParams P{};
+            P.x = in;
+            P.out = out;
+            P.taps64 = filter;
+            P.ns = ns;
+            P.nrow = static_cast<int>(nrow);
+            P.n_even = n_even;
+            P.n_odd = ns / 2;
+            P.wtype = weight_type;
+            P.ds = g.ds;
+            P.dso = g.dso;
+            P.dsd = g.dsd;
+            P.cents = cent_s(g);
+            auto rk = tper <= 2    ? ramp_rows_bf16x3_kernel<kBf16Copies, 2>
+                      : tper <= 4  ? ramp_rows_bf16x3_kernel<kBf16Copies, 4>
+                      : tper <= 6  ? ramp_rows_bf16x3_kernel<kBf16Copies, 6>
+                      : tper <= 8  ? ramp_rows_bf16x3_kernel<kBf16Copies, 8>
+                      : tper <= 10 ? ramp_rows_bf16x3_kernel<kBf16Copies, 10>
+                      : tper <= 12 ? ramp_rows_bf16x3_kernel<kBf16Copies, 12>
+                      : tper <= 14 ? ramp_rows_bf16x3_kernel<kBf16Copies, 14>
+                                   : ramp_rows_bf16x3_kernel<kBf16Copies, 16>;
+            if (rows_lds > 64 * 1024)
+                CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rk),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(rows_lds)));
+            hipLaunchKernelGGL(rk, dim3(div_up(nrow, kRows)), dim3(256), rows_lds, st, P);
+            return check_launch("ramp_rows_bf16x3_kernel");
+        }
+    }
     // scratch: FP32 taps [2ns+1] | weights [ns] | (BF16x3) tap copies [16 nrev] bf16
     const size_t fbytes = ((3 * static_cast<size_t>(ns) + 1) * sizeof(float) + 15) & ~static_cast<size_t>(15);
-    const int nc = bf16_copies();
     const size_t cbytes = bf16x3 ? static_cast<size_t>(2 * nc) * nrev * sizeof(__bf16) : 0;
     unsigned char* scratch = nullptr;
     CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&scratch), fbytes + cbytes, st));
@@ -480,7 +710,7 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
     int rc = check_launch("toeplitz_prep_kernel");
     if (rc == CTR_OK && bf16x3) {
         hipLaunchKernelGGL(bf16_tap_copies_kernel, dim3(div_up(nc * nrev, 256)), dim3(256), 0, st,
-                           hf, copies, ns, nq, nrev, -(ns / 2) - kKC - 4, nc);
+                           filter, copies, ns, nq, nrev, -(ns / 2) - kKC - 4, nc);
         rc = check_launch("bf16_tap_copies_kernel");
     }
     if (rc == CTR_OK) {

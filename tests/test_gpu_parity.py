@@ -319,14 +319,17 @@ def test_autograd_fan_fbp_backward_is_transpose(F, det):
 
 
 # ------------------------------------------- ramp filter on the matrix cores
-@pytest.mark.parametrize("ns", [33, 64, 367, 736, 1472])
+# 37 rows: a partial last row block; 927 / 928 channels sit either side of
+# the whole-row BF16x3 kernel's LDS limit (longer rows take the tiled one).
+@pytest.mark.parametrize("impl", ["mfma", "bf16x3"])
+@pytest.mark.parametrize("ns", [2, 3, 33, 64, 367, 736, 927, 928, 1472])
 @pytest.mark.parametrize("typ,name", [(0, "par"), (1, "fan")])
-def test_ramp_mfma_vs_oracle(F, ns, typ, name):
+def test_ramp_matrix_cores_vs_oracle(F, ns, typ, name, impl):
     nrow = 37
     g = O.geom(ns=ns, nrow=nrow, ds=0.65, dsd=1085.6, type=typ)
     x = randn(ns + typ, (nrow, ns))
     ref = O.ramp(g, x, O.ramp_prep(g))
-    out = host(F.ramp_filter(cuda(x), ns, nrow, 0.65, 1085.6, name, impl="mfma"))
+    out = host(F.ramp_filter(cuda(x), ns, nrow, 0.65, 1085.6, name, impl=impl))
     assert rel_l2(out, ref) < TOL32
 
 
