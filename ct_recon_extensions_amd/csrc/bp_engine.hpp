@@ -368,11 +368,11 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) void bp_forward_kernel(
     const typename Geo::Tables T, const BpLaunch<Geo> L)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    // XCD-aware order of the tiles when the grid is one slice group (the
-    // linear id then equals blockIdx.x).
-    const int ntiles = L.tiles_x * L.tiles_y;
-    const int tile = gridDim.y == 1 ? xcd_tile(blockIdx.x, ntiles) : blockIdx.x;
-    bp_tile<Geo, TW, TH, NB, LROW>(T, L, tile, blockIdx.y * NB, smem);
+    // grid: super_grid(tiles_x, tiles_y) x slice groups (supertile order)
+    int group;
+    const int tile = super_tile(L.tiles_x, L.tiles_y, group);
+    if (tile < 0) return;  // padding of the supertile grid: the whole workgroup
+    bp_tile<Geo, TW, TH, NB, LROW>(T, L, tile, group * NB, smem);
 }
 
 // View-parallel scatter transpose.  One workgroup owns the detector rows of

@@ -150,7 +150,7 @@ int launch_fan_bp_l(BpLaunch<FanGeo<EXACT>> L, const FanTables& T, hipStream_t s
         CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(lds)));
-    dim3 grid(L.tiles_x * L.tiles_y, div_up(L.nbatch, NB));
+    dim3 grid(super_grid(L.tiles_x, L.tiles_y), div_up(L.nbatch, NB));
     hipLaunchKernelGGL(kern, grid, dim3(TS::kBlock), lds, st, T, L);
     return check_launch("bp_forward_kernel<fan>");
 }

@@ -536,7 +536,7 @@ int launch_fpt_engine(const ctr_geom& g, const float* proj, float* img, int nbat
         CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(lds)));
-    hipLaunchKernelGGL(kern, dim3(L.tiles_x * L.tiles_y, div_up(nbatch, NB)),
+    hipLaunchKernelGGL(kern, dim3(super_grid(L.tiles_x, L.tiles_y), div_up(nbatch, NB)),
                        dim3(TileShape<TW, TH>::kBlock), lds, st, T, L);
     return check_launch("bp_forward_kernel<fp transpose>");
 }

@@ -158,7 +158,7 @@ int launch_par_bp_l(BpLaunch<ParGeo> L, const ParTables& T, hipStream_t st)
         CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(lds)));
-    dim3 grid(L.tiles_x * L.tiles_y, div_up(L.nbatch, NB));
+    dim3 grid(super_grid(L.tiles_x, L.tiles_y), div_up(L.nbatch, NB));
     hipLaunchKernelGGL(kern, grid, dim3(TS::kBlock), lds, st, T, L);
     return check_launch("bp_forward_kernel<par>");
 }

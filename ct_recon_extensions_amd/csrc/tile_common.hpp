@@ -41,6 +41,33 @@ inline int engine_nb(int nb, long long na, long long ns)
     return (nb > 2 && na * ns * 4 >= (1LL << 29)) ? 2 : nb;
 }
 
+// Tile order of the BP engine's forward kernel: supertiles of kSuper x
+// kSuper tiles, row-major, tiles row-major inside each, and every XCD given a
+// contiguous run of that order (xcd_tile over all slice groups).  An XCD then
+// holds a compact block of the image, whose detector spans overlap far more
+// than those of a band of whole tile rows, so more of its staging hits its
+// own L2.  The grid is padded to whole supertiles; padding workgroups get -1.
+constexpr int kSuper = 8;
+
+__host__ __device__ inline int super_grid(int tiles_x, int tiles_y)
+{
+    return ((tiles_x + kSuper - 1) / kSuper) * ((tiles_y + kSuper - 1) / kSuper) * kSuper * kSuper;
+}
+
+// Workgroup (bx, by) of a grid of super_grid(...) x gridDim.y -> tile index
+// (or -1) and slice group.
+__device__ __forceinline__ int super_tile(int tiles_x, int tiles_y, int& group)
+{
+    const int g = gridDim.x;
+    const int l = xcd_tile(blockIdx.x + blockIdx.y * g, g * gridDim.y);
+    group = l / g;
+    const int w = l - group * g;
+    const int sx = (tiles_x + kSuper - 1) / kSuper;
+    const int s = w / (kSuper * kSuper), i = w - s * (kSuper * kSuper);
+    const int tx = (s % sx) * kSuper + i % kSuper, ty = (s / sx) * kSuper + i / kSuper;
+    return (tx < tiles_x && ty < tiles_y) ? ty * tiles_x + tx : -1;
+}
+
 // Origin of a tile and the pixel (ic, jc) its anchor is evaluated at.
 struct TileFrame {
     int x0, y0, ic, jc;
