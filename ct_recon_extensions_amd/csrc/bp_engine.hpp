@@ -282,6 +282,22 @@ __device__ __forceinline__ void bp_chunk(const BpLaunch<Geo>& L,
     }
 }
 
+// Pixel (px, py) of a TW-wide tile handled by thread index p.  A wave's 64
+// lanes cover TW x (64 / TW) pixels, and each 16-lane quarter (the lanes one
+// 16-byte LDS read serves per pass) takes a compact block: 4 x 4 pixels
+// (8 x 2 for TW = 32) rather than a 16-pixel row, so the channels its taps
+// touch stay within 16 of each other and do not collide on LDS banks.
+template <int TW>
+__device__ __forceinline__ void tile_pixel(int p, int& px, int& py)
+{
+    constexpr int kRowsPerWave = 64 / TW;  // TW in {8, 16, 32}
+    constexpr int kBw = kRowsPerWave >= 4 ? 4 : 8, kBh = 16 / kBw;
+    const int wave = p >> 6, l = p & 63, q = l >> 4, w = l & 15;
+    constexpr int kBlocksX = TW / kBw;
+    px = (q % kBlocksX) * kBw + (w % kBw);
+    py = wave * kRowsPerWave + (q / kBlocksX) * kBh + w / kBw;
+}
+
 // One pixel tile x NB slices (starting at slice b0) over all views: the
 // whole forward computation of one workgroup.  smem holds kAnchorChunk
 // anchors followed by two span buffers of vchunk x NB x LROW floats.  The
@@ -310,9 +326,10 @@ __device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpL
     float total[PPT][NB];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        const int p = tid + k * BLOCK;
-        fi[k] = static_cast<float>(F.x0 + p % TW - F.ic);
-        fj[k] = static_cast<float>(F.y0 + p / TW - F.jc);
+        int px, py;
+        tile_pixel<TW>(tid + k * BLOCK, px, py);
+        fi[k] = static_cast<float>(F.x0 + px - F.ic);
+        fj[k] = static_cast<float>(F.y0 + py - F.jc);
 #pragma unroll
         for (int b = 0; b < NB; ++b) total[k][b] = 0.0f;
     }
@@ -350,8 +367,9 @@ __device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpL
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        const int p = tid + k * BLOCK;
-        const int ix = F.x0 + p % TW, iy = F.y0 + p / TW;
+        int px, py;
+        tile_pixel<TW>(tid + k * BLOCK, px, py);
+        const int ix = F.x0 + px, iy = F.y0 + py;
         if (ix < P.nx && iy < P.ny) {
 #pragma unroll
             for (int b = 0; b < NB; ++b)
