@@ -329,8 +329,8 @@ def bench_filter_precision(torch, reps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="host threads for the CPU baseline (default: min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -384,12 +384,13 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    ms_step = time_events(step, args.steps, torch)
-    print(f"[bench] step {ms_step:.3f} ms", file=sys.stderr, flush=True)
-
-    # per-kernel durations on the stream the kernels are launched on
+    # per-kernel durations on the stream the kernels are launched on; they run
+    # before the step is timed, so the GPU has also left its idle clocks
+    # (after seconds of host-side input generation) by then
     ms_filter = time_events(lambda: F.fan_filter(proj, **kwf), args.steps, torch)
     ms_bp = time_events(lambda: F.fan_backprojection_2d(filt, **kwb), args.steps, torch)
+    ms_step = time_events(step, args.steps, torch)
+    print(f"[bench] step {ms_step:.3f} ms", file=sys.stderr, flush=True)
 
     B, n, na, ns = cfg["nbatch"], cfg["n"], cfg["na"], cfg["ns"]
     pv_slice = n * n * na

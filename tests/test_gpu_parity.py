@@ -252,6 +252,26 @@ def test_fp_non_square_image(F):
         assert rel_l2(out[b], O.fp(g, img[b], t)) < TOL32
 
 
+# Batches of 3 .. 8 slices take the four-slice kernel (a partial last group
+# at 3, 5, 7); the second geometry is non-square with offsets, and its wide
+# detector sends rays past the image on both sides.
+@pytest.mark.parametrize("nbatch", [3, 4, 5, 7, 8])
+@pytest.mark.parametrize("geo", [
+    dict(ns=91, na=90, ds=1.0, orbit=math.pi / 90, offset_s=0.0, orbit_start=0.0, nx=64, ny=64,
+         dx=1.0, dy=1.0, offset_x=0.0, offset_y=0.0),
+    dict(ns=140, na=47, ds=0.8, orbit=math.pi / 47, offset_s=0.35, orbit_start=0.2, nx=50, ny=36,
+         dx=1.1, dy=0.9, offset_x=-0.6, offset_y=0.45),
+], ids=["square", "offset"])
+def test_fp_batches_vs_oracle(F, geo, nbatch):
+    g = O.geom(**geo)
+    img = randn(20 + nbatch, (nbatch, g.ny, g.nx))
+    kw = par_kwargs(dict(geo, dso=0, dsd=0))
+    out = host(F.parallel_projection_2d(cuda(img), **kw))
+    t = O.fp_prep(g)
+    for b in range(nbatch):
+        assert rel_l2(out[b], O.fp(g, img[b], t)) < TOL32, b
+
+
 # ----------------------------------------------------- filter / weighting
 @pytest.mark.parametrize("typ,name", [(0, "par"), (1, "fan"), (2, "flat")])
 def test_ramp_golden(F, typ, name):
