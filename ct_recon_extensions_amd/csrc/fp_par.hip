@@ -406,7 +406,7 @@ int launch_fp(const FpProblem& P, const T* img, const T* imgT, T* proj, const do
     return check_launch("fp_kernel");
 }
 
-constexpr int kFpViews = 2;
+constexpr int kFpViews = 4;
 constexpr int kFpSlices = 4;
 // 4-row chunks with 512-column spans (32 KB of LDS at 4 slices, 5 workgroups
 // per CU) measured 0.88 ms on the 8 x 512^2, 720-view scan against 0.95 ms
