@@ -406,9 +406,10 @@ int launch_fp(const FpProblem& P, const T* img, const T* imgT, T* proj, const do
     return check_launch("fp_kernel");
 }
 
-// Views per workgroup: 4 measured 0.83 ms on the 8 x 512^2, 720-view scan,
-// 2 0.88 ms (each staged row chunk serves half as many rays) and 8 1.26 ms
-// (234 VGPRs: two waves per SIMD).
+// Views per workgroup, at most kFpViews: 4 measured 0.83 ms on the
+// 8 x 512^2, 720-view scan, 2 0.88 ms (each staged row chunk serves half as
+// many rays) and 8 1.26 ms (234 VGPRs: two waves per SIMD).  Small scans take
+// fewer (fp_views) so that the grid still fills the GPU.
 constexpr int kFpViews = 4;
 constexpr int kFpSlices = 4;
 // 4-row chunks with 512-column spans (32 KB of LDS at 4 slices, 5 workgroups
@@ -419,9 +420,10 @@ constexpr int kFpRowsDefault = 4, kFpSpanDefault = 512;
 // Columns the rays of one fp_lds_kernel chunk can span: cpb channels at up
 // to sqrt(2) ds / pixel per channel, kFpRows rows at up to one column per row
 // (in pixel units of the marched axis), and the drift of a ray's column
-// across kFpViews views (at most 4 R |orbit| per view for an image of
+// across `views` views (at most 4 R |orbit| per view for an image of
 // half-diagonal R pixels), plus margins.
-bool fp_lds_applies(const ctr_geom& g, int rows = kFpRowsDefault, int span_max = kFpSpanDefault)
+bool fp_lds_applies(const ctr_geom& g, int views, int rows = kFpRowsDefault,
+                    int span_max = kFpSpanDefault)
 {
     const double pmin = std::min(std::fabs(g.dx), std::fabs(g.dy));
     const double pmax = std::max(std::fabs(g.dx), std::fabs(g.dy));
@@ -430,21 +432,41 @@ bool fp_lds_applies(const ctr_geom& g, int rows = kFpRowsDefault, int span_max =
                                       g.ny + 2 * std::fabs(g.offset_y)) * pmax / pmin;
     const int cpb = div_up(g.ns, div_up(g.ns, 256));  // channels per workgroup
     const double span = (cpb - 1) * std::sqrt(2.0) * g.ds / pmin + rows * pmax / pmin +
-                        4.0 * r * kFpViews * std::fabs(g.orbit) + 8.0;
+                        4.0 * r * views * std::fabs(g.orbit) + 8.0;
     return span + 2.0 <= span_max;
 }
 
-template <int NB, int R = kFpRowsDefault, int SPAN = kFpSpanDefault>
+// Views per workgroup for nb slices per workgroup: the most (up to
+// kFpViews) that leave at least 4 workgroups per CU (1024), else 1.
+int fp_views(const ctr_geom& g, int nbatch, int nb)
+{
+    const long long blocks = static_cast<long long>(div_up(g.ns, 256)) * div_up(nbatch, nb);
+    for (int v = kFpViews; v > 1; v /= 2)
+        if (blocks * div_up(g.na, v) >= 1024) return v;
+    return 1;
+}
+
+template <int NB, int V, int R = kFpRowsDefault, int SPAN = kFpSpanDefault>
 int launch_fp_lds(const FpProblem& P, const float* img, const float* imgT, float* proj,
                   const double* sc, const double* bo, const int* usex, hipStream_t st)
 {
     // channels split evenly over ceil(ns / 256) workgroups
     const int nblk = div_up(P.ns, 256);
     const int cpb = div_up(P.ns, nblk);
-    dim3 grid(nblk, div_up(P.na, kFpViews), div_up(P.nbatch, NB));
-    hipLaunchKernelGGL((fp_lds_kernel<NB, kFpViews, R, SPAN>), grid, dim3(256), 0, st, img, imgT,
-                       proj, sc, bo, usex, P, cpb);
+    dim3 grid(nblk, div_up(P.na, V), div_up(P.nbatch, NB));
+    hipLaunchKernelGGL((fp_lds_kernel<NB, V, R, SPAN>), grid, dim3(256), 0, st, img, imgT, proj,
+                       sc, bo, usex, P, cpb);
     return check_launch("fp_lds_kernel");
+}
+
+template <int NB>
+int launch_fp_lds_v(int views, const FpProblem& P, const float* img, const float* imgT,
+                    float* proj, const double* sc, const double* bo, const int* usex,
+                    hipStream_t st)
+{
+    return views >= 4 ? launch_fp_lds<NB, 4>(P, img, imgT, proj, sc, bo, usex, st)
+           : views >= 2 ? launch_fp_lds<NB, 2>(P, img, imgT, proj, sc, bo, usex, st)
+                        : launch_fp_lds<NB, 1>(P, img, imgT, proj, sc, bo, usex, st);
 }
 
 template <typename T>
@@ -461,10 +483,12 @@ int fp_typed(const ctr_geom& g, const T* img, T* proj, int nbatch, const double*
                        static_cast<int>(g.nx));
     int rc = check_launch("transpose_kernel");
     if constexpr (std::is_same<T, float>::value) {
-        if (rc == CTR_OK && fp_lds_applies(g)) {
-            rc = nbatch >= kFpSlices ? launch_fp_lds<kFpSlices>(P, img, imgT, proj, sc, bo, usex, st)
-                 : nbatch >= 2       ? launch_fp_lds<2>(P, img, imgT, proj, sc, bo, usex, st)
-                                     : launch_fp_lds<1>(P, img, imgT, proj, sc, bo, usex, st);
+        const int nb = nbatch >= kFpSlices ? kFpSlices : nbatch >= 2 ? 2 : 1;
+        const int views = fp_views(g, nbatch, nb);
+        if (rc == CTR_OK && fp_lds_applies(g, views)) {
+            rc = nb == kFpSlices ? launch_fp_lds_v<kFpSlices>(views, P, img, imgT, proj, sc, bo, usex, st)
+                 : nb == 2       ? launch_fp_lds_v<2>(views, P, img, imgT, proj, sc, bo, usex, st)
+                                 : launch_fp_lds_v<1>(views, P, img, imgT, proj, sc, bo, usex, st);
             CTR_HIP(hipFreeAsync(imgT, st));
             return rc;
         }
