@@ -40,12 +40,24 @@ __device__ __forceinline__ float uniform_f(float x)
 }
 __device__ __forceinline__ int uniform_i(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Views per LDS round for the forward kernel: the most (up to one anchor
+// chunk) whose two span buffers and the anchors fit a quarter of a CU's
+// 160 KB, i.e. four workgroups per CU (fewer views per round means a barrier
+// more often; more LDS per workgroup, fewer workgroups to hide latency).
+template <class Anchor>
+inline int engine_vchunk(int nb, int lrow)
+{
+    const int avail = 160 * 1024 / 4 - kAnchorChunk * static_cast<int>(sizeof(Anchor));
+    const int v = avail / (2 * nb * lrow * static_cast<int>(sizeof(float)));
+    return v < 1 ? 1 : v > kAnchorChunk ? kAnchorChunk : v;
+}
+
 template <class Geo>
 struct BpLaunch {
     typename Geo::Problem P;
     int nbatch, tiles_x, tiles_y;
     int lmax;    // channels a (view, slice) span needs (<= LROW)
-    int vchunk;  // views per LDS buffer
+    int vchunk;  // views per LDS buffer (0: engine_vchunk)
     double scale;
     const float* proj;
     float* img;
@@ -310,7 +322,8 @@ __device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpL
     using TS = TileShape<TW, TH>;
     using Anchor = typename Geo::Anchor;
     constexpr int BLOCK = TS::kBlock, PPT = TS::kPPT, NWAVES = BLOCK / 64;
-    static_assert(LROW % 64 == 0, "span rows are whole waves of dwords");
+    static_assert(NB % 4 == 0 ? LROW % 16 == 0 : LROW % 64 == 0,
+                  "span rows are whole DMA rows (16 channels x 4 slices, or 64 channels)");
     Anchor* anchors = reinterpret_cast<Anchor*>(smem);
     float* spans = reinterpret_cast<float*>(smem + kAnchorChunk * sizeof(Anchor));
     const int bufsz = L.vchunk * NB * LROW;  // floats per buffer

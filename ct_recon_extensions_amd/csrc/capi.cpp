@@ -93,8 +93,8 @@ int ctr_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nb
 int ctr_fan_bp_config(int tile, int vchunk, int nb)
 {
     CTR_REQUIRE(tile >= 0 && tile <= 2, CTR_ERR_ARG, "ctr_fan_bp_config: tile must be 0, 1 or 2");
-    CTR_REQUIRE(vchunk >= 1 && vchunk <= 64, CTR_ERR_ARG,
-                "ctr_fan_bp_config: vchunk must be in [1, 64]");
+    CTR_REQUIRE(vchunk >= 0 && vchunk <= 64, CTR_ERR_ARG,
+                "ctr_fan_bp_config: vchunk must be in [0, 64]");
     CTR_REQUIRE(nb == 0 || nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16, CTR_ERR_ARG,
                 "ctr_fan_bp_config: nb must be 0, 1, 2, 4, 8 or 16");
     ctr::fan_bp_set_config(tile, vchunk, nb);

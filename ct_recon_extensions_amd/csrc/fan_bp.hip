@@ -140,6 +140,7 @@ int launch_fan_bp_l(BpLaunch<FanGeo<EXACT>> L, const FanTables& T, hipStream_t s
 {
     using TS = TileShape<TW, TH>;
     // two span buffers of vchunk x NB rows of LROW floats must fit in LDS
+    if (L.vchunk <= 0) L.vchunk = engine_vchunk<FanAnchor>(NB, LROW);
     L.vchunk = std::max(1, std::min(L.vchunk, (150 * 1024 / 8) / (NB * LROW)));
     const size_t lds = kAnchorChunk * sizeof(FanAnchor) +
                        2 * static_cast<size_t>(L.vchunk) * NB * LROW * sizeof(float);
@@ -158,6 +159,10 @@ int launch_fan_bp_l(BpLaunch<FanGeo<EXACT>> L, const FanTables& T, hipStream_t s
 template <int TW, int TH, int NB, bool EXACT>
 int launch_fan_bp(const BpLaunch<FanGeo<EXACT>>& L, const FanTables& T, hipStream_t st)
 {
+    // slice-interleaved spans come in 16-channel rows: a 48-channel row leaves
+    // LDS for more views per round than 64
+    if constexpr (NB % 4 == 0)
+        if (L.lmax <= 48) return launch_fan_bp_l<TW, TH, NB, EXACT, 48>(L, T, st);
     if (L.lmax <= 64) return launch_fan_bp_l<TW, TH, NB, EXACT, 64>(L, T, st);
     if (L.lmax <= 128) return launch_fan_bp_l<TW, TH, NB, EXACT, 128>(L, T, st);
     if (L.lmax <= 256) return launch_fan_bp_l<TW, TH, NB, EXACT, 256>(L, T, st);
@@ -200,12 +205,13 @@ double fan_bp_scale(const ctr_geom& g)
     return std::fabs(g.orbit) / factor;
 }
 
-// Tunables of the forward kernel (set by the autotuner through
-// ctr_fan_bp_config): tile 0 = 16x16, 1 = 8x8, 2 = 32x8; views per LDS round;
-// slices per workgroup (0 = automatic).
+// Tunables of the forward kernel (set through ctr_fan_bp_config): tile
+// 0 = 16x16, 1 = 8x8, 2 = 32x8; views per LDS round (0 = engine_vchunk: on
+// the 16-slice 512^2 fan scan 6 views of 48-channel rows, 0.418 ms against
+// 0.453 for 4 views of 64); slices per workgroup (0 = automatic).
 struct FanBpConfig {
     int tile = 0;
-    int vchunk = 4;
+    int vchunk = 0;
     int nb = 0;
 };
 static FanBpConfig g_fan_cfg;
@@ -213,7 +219,7 @@ static FanBpConfig g_fan_cfg;
 void fan_bp_set_config(int tile, int vchunk, int nb)
 {
     g_fan_cfg.tile = (tile >= 0 && tile <= 2) ? tile : 0;
-    g_fan_cfg.vchunk = (vchunk > 0 && vchunk <= kAnchorChunk) ? vchunk : 4;
+    g_fan_cfg.vchunk = (vchunk > 0 && vchunk <= kAnchorChunk) ? vchunk : 0;
     g_fan_cfg.nb = (nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16) ? nb : 0;
 }
 

@@ -538,12 +538,12 @@ int launch_fpt_border(const FpProblem& P, const float* proj, float* img, const d
     return rc;
 }
 
-template <int NB>
+template <int NB, int LROW>
 int launch_fpt_engine(const ctr_geom& g, const float* proj, float* img, int nbatch,
                       const double* w, const double* pos, const int* usex, int stride,
                       hipStream_t st)
 {
-    constexpr int TW = 16, TH = 16, LROW = 64;
+    constexpr int TW = 16, TH = 16;
     BpLaunch<FptGeo> L{};
     L.P = FptProblem{static_cast<int>(g.ns), static_cast<int>(g.na), static_cast<int>(g.nx),
                      static_cast<int>(g.ny)};
@@ -551,7 +551,7 @@ int launch_fpt_engine(const ctr_geom& g, const float* proj, float* img, int nbat
     L.tiles_x = div_up(g.nx, TW);
     L.tiles_y = div_up(g.ny, TH);
     L.lmax = LROW;
-    L.vchunk = 4;  // 35.8 KB of LDS at NB = 8: four workgroups per CU
+    L.vchunk = engine_vchunk<FptAnchor>(NB, LROW);
     L.scale = 1.0;
     L.proj = proj;
     L.img = img;
@@ -587,10 +587,13 @@ int fpt_typed(const ctr_geom& g, const T* proj, T* img, int nbatch, const double
             const int s = static_cast<int>(stride);
             int rc;
             const int nb = engine_nb(nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch, g.na, g.ns);
-            if (nb >= 8) rc = launch_fpt_engine<8>(g, proj, img, nbatch, w, pos, usex, s, st);
-            else if (nb >= 4) rc = launch_fpt_engine<4>(g, proj, img, nbatch, w, pos, usex, s, st);
-            else if (nb >= 2) rc = launch_fpt_engine<2>(g, proj, img, nbatch, w, pos, usex, s, st);
-            else rc = launch_fpt_engine<1>(g, proj, img, nbatch, w, pos, usex, s, st);
+            // the applicability bound keeps a tile-view's span within
+            // 30 max(dx,dy)/ds + 5 <= 35 channels: 48-channel rows when the
+            // slices are interleaved four to a word, 64 otherwise
+            if (nb >= 8) rc = launch_fpt_engine<8, 48>(g, proj, img, nbatch, w, pos, usex, s, st);
+            else if (nb >= 4) rc = launch_fpt_engine<4, 48>(g, proj, img, nbatch, w, pos, usex, s, st);
+            else if (nb >= 2) rc = launch_fpt_engine<2, 64>(g, proj, img, nbatch, w, pos, usex, s, st);
+            else rc = launch_fpt_engine<1, 64>(g, proj, img, nbatch, w, pos, usex, s, st);
             if (rc) return rc;
             FpProblem P{static_cast<int>(g.ns), static_cast<int>(g.na), static_cast<int>(g.nx),
                         static_cast<int>(g.ny), nbatch, g.dx, g.dy};

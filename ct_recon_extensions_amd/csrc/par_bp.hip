@@ -149,6 +149,7 @@ template <int TW, int TH, int NB, int LROW>
 int launch_par_bp_l(BpLaunch<ParGeo> L, const ParTables& T, hipStream_t st)
 {
     using TS = TileShape<TW, TH>;
+    if (L.vchunk <= 0) L.vchunk = engine_vchunk<ParAnchor>(NB, LROW);
     L.vchunk = std::max(1, std::min(L.vchunk, (150 * 1024 / 8) / (NB * LROW)));
     const size_t lds = kAnchorChunk * sizeof(ParAnchor) +
                        2 * static_cast<size_t>(L.vchunk) * NB * LROW * sizeof(float);
@@ -166,6 +167,8 @@ int launch_par_bp_l(BpLaunch<ParGeo> L, const ParTables& T, hipStream_t st)
 template <int TW, int TH, int NB>
 int launch_par_bp(const BpLaunch<ParGeo>& L, const ParTables& T, hipStream_t st)
 {
+    if constexpr (NB % 4 == 0)
+        if (L.lmax <= 48) return launch_par_bp_l<TW, TH, NB, 48>(L, T, st);
     if (L.lmax <= 64) return launch_par_bp_l<TW, TH, NB, 64>(L, T, st);
     if (L.lmax <= 128) return launch_par_bp_l<TW, TH, NB, 128>(L, T, st);
     if (L.lmax <= 256) return launch_par_bp_l<TW, TH, NB, 256>(L, T, st);
@@ -202,7 +205,7 @@ int par_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch
     L.nbatch = nbatch;
     L.tiles_x = div_up(g.nx, TW);
     L.tiles_y = div_up(g.ny, TH);
-    L.vchunk = 4;
+    L.vchunk = 0;  // engine_vchunk
     L.scale = g.orbit;
     if (dtype == CTR_F64) {
         hipLaunchKernelGGL(par_bp_f64_kernel, dim3(div_up(g.nx * g.ny, 256), nbatch), dim3(256),

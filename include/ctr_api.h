@@ -104,8 +104,9 @@ int ctr_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nb
                void* stream);
 /* Launch configuration of ctr_fan_bp's float32 kernel for the calling
  * process: tile 0 = 16x16, 1 = 8x8, 2 = 32x8 pixels; vchunk = views per LDS
- * round (1..64, capped to the LDS budget); nb = slices per workgroup
- * (1, 2, 4, 8, 16; 0 = from the batch).  Defaults 0, 4, 0.  Tiles whose
+ * round (1..64, capped to the LDS budget; 0 = the most that keep four
+ * workgroups per CU); nb = slices per workgroup (1, 2, 4, 8, 16; 0 = from
+ * the batch).  Defaults 0, 0, 0.  Tiles whose
  * detector span exceeds 256 channels switch to 8x8 by themselves.  No
  * counterpart in the reference, whose launch shape is fixed
  * (cuda/bp_fan_2d.cu:219-245). */

@@ -128,6 +128,15 @@ def test_error_paths_return_codes(lib):
     assert rc == 1 and "weight_type" in _lib.last_error()
 
 
+def test_fan_bp_config_validates_arguments(lib):
+    from ct_recon_extensions_amd import _lib
+    for bad, word in (((3, 0, 0), "tile"), ((0, -1, 0), "vchunk"), ((0, 65, 0), "vchunk"),
+                      ((0, 0, 3), "nb")):
+        assert lib.ctr_fan_bp_config(*bad) == 1 and word in _lib.last_error(), bad
+    assert lib.ctr_fan_bp_config(2, 6, 8) == 0
+    assert lib.ctr_fan_bp_config(0, 0, 0) == 0  # back to the defaults
+
+
 def test_python_ops_refuse_cpu_tensors(lib):
     import torch
     import ct_recon_extensions_amd as F

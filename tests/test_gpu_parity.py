@@ -97,6 +97,29 @@ def test_fan_bp_batches_match_per_slice_oracle(F, nbatch):
         assert rel_l2(out[b], O.fan_bp(g, proj[b], t)) < TOL32, b
 
 
+# The launch shape (tile, views per LDS round, slices per workgroup) does not
+# change the result beyond float32 rounding: every configuration against the
+# oracle, including view chunks past the LDS budget (capped) and 48- and
+# 64-channel span rows.
+@pytest.mark.parametrize("tile,vchunk,nb", [(0, 0, 0), (0, 1, 4), (0, 6, 16), (0, 64, 8),
+                                            (1, 0, 0), (2, 5, 16), (1, 3, 2)])
+def test_fan_bp_launch_configs_match_oracle(F, tile, vchunk, nb):
+    from ct_recon_extensions_amd import _lib
+    from tests.golden.make_golden import FAN_SMALL
+    geo = dict(FAN_SMALL, fov=-1.0)
+    g = O.geom(**FAN_SMALL)
+    t = O.fan_prep(g)
+    proj = randn(700 + vchunk, (16, g.na, g.ns))
+    lib = _lib.load()
+    try:
+        assert lib.ctr_fan_bp_config(tile, vchunk, nb) == 0
+        out = host(F.fan_backprojection_2d(cuda(proj), **fan_kwargs(geo)))
+    finally:
+        lib.ctr_fan_bp_config(0, 0, 0)
+    for b in range(16):
+        assert rel_l2(out[b], O.fan_bp(g, proj[b], t)) < TOL32, b
+
+
 def test_fan_bp_float64(F):
     geo, d = load_golden("fan_small")
     out = host(F.fan_backprojection_2d(cuda(d["proj"][None], torch.float64), **fan_kwargs(geo)))
