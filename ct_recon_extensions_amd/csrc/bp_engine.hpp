@@ -191,9 +191,11 @@ __device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
     } else {
         float f, w;
         Geo::pixel(P, A, fi, fj, f, w);
-        const float fl = floorf(f);
-        const float fr = f - fl;
-        idx = static_cast<int>(fl);
+        // f >= 1 for every pixel of the tile (is0 sits a channel below the
+        // tile's lowest position): truncation is the floor, and v_fract_f32
+        // gives f - floor(f) exactly
+        const float fr = __builtin_amdgcn_fractf(f);
+        idx = static_cast<int>(f);
         if (__builtin_expect(!inside, 0)) w = on_detector(is0 + idx, fr, P.ns) ? w : 0.0f;
         idx = min(max(idx, 0), LROW - 2);
         w1 = w * fr;

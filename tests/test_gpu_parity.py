@@ -157,6 +157,14 @@ def test_fan_bp_full_size_slice_vs_oracle(F):
     err = rel_l2(out[0], ref)
     print(f"fan BP 512^2 x 720 views, rel L2 vs oracle: {err:.3e}")
     assert err < TOL32
+    # 16 slices (the batched kernel: slice-interleaved 48-channel spans):
+    # power-of-two multiples of the same sinogram, exact in float32, so each
+    # slice must be its multiple of the single-slice oracle result
+    scales = 2.0 ** (np.arange(16) - 8)
+    batch = (scales[:, None, None] * filt[None]).astype(np.float32)
+    outb = host(F.fan_backprojection_2d(cuda(batch), **fan_kwargs(geo)))
+    for b in range(16):
+        assert rel_l2(outb[b], scales[b] * ref) < TOL32, b
 
 
 # ------------------------------------------------------- fan BP transpose
