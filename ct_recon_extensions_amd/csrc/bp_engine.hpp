@@ -180,10 +180,26 @@ struct HasCustomTaps {
     static constexpr bool value = test<Geo>(nullptr);
 };
 
-template <class Geo, int LROW>
+// Optional Geo::pin(Problem&): moves the per-pixel constants the policy
+// combines with another uniform operand into VGPRs once per workgroup (a
+// VALU instruction reads at most one SGPR, so otherwise each use costs a
+// v_mov per pixel-view).
+template <class Geo>
+struct HasPin {
+    template <class G>
+    static constexpr bool test(decltype(&G::pin)) { return true; }
+    template <class G>
+    static constexpr bool test(...) { return false; }
+    static constexpr bool value = test<Geo>(nullptr);
+};
+
+// EDGE: the view's tile may reach past the detector (anchor field inside
+// = 0), so the weight of a tap off [0, ns-1] is zeroed; chunks whose views
+// all have inside = 1 skip the test.
+template <class Geo, int LROW, bool EDGE>
 __device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
-                                            const typename Geo::Anchor& A, int inside, int is0,
-                                            float fi, float fj, int& idx, float& w0, float& w1)
+                                            const typename Geo::Anchor& A, float fi, float fj,
+                                            int& idx, float& w0, float& w1)
 {
     if constexpr (HasCustomTaps<Geo>::value) {
         Geo::taps(P, A, fi, fj, idx, w0, w1);
@@ -196,11 +212,87 @@ __device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
         // gives f - floor(f) exactly
         const float fr = __builtin_amdgcn_fractf(f);
         idx = static_cast<int>(f);
-        if (__builtin_expect(!inside, 0)) w = on_detector(is0 + idx, fr, P.ns) ? w : 0.0f;
+        if constexpr (EDGE) {
+            const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
+            if (!inside) w = on_detector(is0 + idx, fr, P.ns) ? w : 0.0f;
+        }
         idx = min(max(idx, 0), LROW - 2);
         w1 = w * fr;
         w0 = w - w1;
     }
+}
+
+// Taps and weights of one pixel-view: LDS word index of the left tap and
+// the two weights, for each of a thread's PPT pixels.
+template <int PPT>
+struct ViewTaps {
+    int idx[PPT];
+    float w0[PPT], w1[PPT];
+};
+
+// One view of the software pipeline: read the taps of this view (cur,
+// computed one step earlier) from sview, compute the next view's taps into
+// nxt while the reads are in flight (when has_next), then accumulate.  The
+// caller alternates two ViewTaps, so nothing is copied between steps.
+template <class Geo, int NB, int LROW, int PPT, bool EDGE>
+__device__ __forceinline__ void bp_view(const typename Geo::Problem& Pg,
+                                        const typename Geo::Anchor* an_next, bool has_next,
+                                        const float* sview, const float (&fi)[PPT],
+                                        const float (&fj)[PPT], const ViewTaps<PPT>& cur,
+                                        ViewTaps<PPT>& nxt, float (&acc)[PPT][NB])
+{
+    using Anchor = typename Geo::Anchor;
+    // LDS returns in order: the next view's anchor is read before this
+    // view's taps (the empty asm keeps the compiler from sinking the read
+    // below them), so the next geometry need not wait for the taps.
+    const Anchor An = *an_next;
+    asm volatile("" ::: "memory");
+    float p0[PPT][NB], p1[PPT][NB];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        if constexpr (NB % 4 == 0) {
+            // [group][channel][4 slices]: one ds_read_b128 per tap
+            // and group, all at immediate offsets from one address
+            const float4* q = reinterpret_cast<const float4*>(sview) + cur.idx[k];
+#pragma unroll
+            for (int g = 0; g < NB / 4; ++g) {
+                const float4 a = q[g * LROW], c = q[g * LROW + 1];
+                p0[k][4 * g] = a.x;
+                p0[k][4 * g + 1] = a.y;
+                p0[k][4 * g + 2] = a.z;
+                p0[k][4 * g + 3] = a.w;
+                p1[k][4 * g] = c.x;
+                p1[k][4 * g + 1] = c.y;
+                p1[k][4 * g + 2] = c.z;
+                p1[k][4 * g + 3] = c.w;
+            }
+        } else {
+            // Right taps come through a second base whose +1 the
+            // compiler cannot see, so the loads pair up per tap
+            // across slices (rows LROW words apart:
+            // ds_read2st64_b32, no per-slice address arithmetic).
+            int one = 1;
+            asm volatile("" : "+v"(one));
+            const float* q0 = sview + cur.idx[k];
+            const float* q1 = sview + cur.idx[k] + one;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                p0[k][b] = q0[b * LROW];
+                p1[k][b] = q1[b * LROW];
+            }
+        }
+    }
+    if (has_next) {
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+            tap_weights<Geo, LROW, EDGE>(Pg, An, fi[k], fj[k], nxt.idx[k], nxt.w0[k],
+                                         nxt.w1[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; ++k)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            acc[k][b] = fmaf(cur.w1[k], p1[k][b], fmaf(cur.w0[k], p0[k][b], acc[k][b]));
 }
 
 // One chunk of views of a tile: issue the DMA of the next chunk into `next`,
@@ -208,8 +300,8 @@ __device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
 // three LDS regions are disjoint, and the restrict qualifiers say so: the
 // compiler then knows the span reads cannot depend on the DMA still in
 // flight into `next` and inserts no wait for it.
-template <class Geo, int NB, int LROW, int NWAVES, int PPT>
-__device__ __forceinline__ void bp_chunk(const BpLaunch<Geo>& L,
+template <class Geo, int NB, int LROW, int NWAVES, int PPT, bool EDGE>
+__device__ __forceinline__ void bp_chunk(const BpLaunch<Geo>& L, const typename Geo::Problem& Pg,
                                          const typename Geo::Anchor* __restrict__ anchors,
                                          float* __restrict__ next, const float* __restrict__ sbuf,
                                          bool prefetch, int va, int vc, int nv, int nv_next,
@@ -217,82 +309,52 @@ __device__ __forceinline__ void bp_chunk(const BpLaunch<Geo>& L,
                                          const float (&fi)[PPT], const float (&fj)[PPT],
                                          float (&acc)[PPT][NB])
 {
-    using Anchor = typename Geo::Anchor;
-    const auto& P = L.P;
     if (prefetch)
         stage_views<Geo, NB, LROW, NWAVES>(L, anchors, next, va + vc + L.vchunk, vc + L.vchunk,
                                            nv_next, b0, nb, wave, lane);
-    // Software pipeline over the views of the chunk: the taps of view
-    // t are read while the geometry of view t+1 is computed.
-    int idx[PPT];
-    float w0[PPT], w1[PPT];
-    {
-        const Anchor A = anchors[vc];
-        const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
+    // Software pipeline over the views of the chunk, two views per trip:
+    // the taps of view t are read while the geometry of view t+1 is computed.
+    ViewTaps<PPT> ta, tb;
 #pragma unroll
-        for (int k = 0; k < PPT; ++k)
-            tap_weights<Geo, LROW>(P, A, inside, is0, fi[k], fj[k], idx[k], w0[k], w1[k]);
+    for (int k = 0; k < PPT; ++k) {
+        tap_weights<Geo, LROW, EDGE>(Pg, anchors[vc], fi[k], fj[k], ta.idx[k], ta.w0[k],
+                                     ta.w1[k]);
+        tb.idx[k] = 0;
+        tb.w0[k] = tb.w1[k] = 0.0f;
     }
-    for (int t = 0; t < nv; ++t) {
-        const float* sview = sbuf + t * NB * LROW;
-        // LDS returns in order: the next view's anchor is read before this
-        // view's taps (the empty asm keeps the compiler from sinking the read
-        // below them), so the next geometry need not wait for the taps.
-        const Anchor An = anchors[vc + min(t + 1, nv - 1)];
-        asm volatile("" ::: "memory");
-        float p0[PPT][NB], p1[PPT][NB];
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            if constexpr (NB % 4 == 0) {
-                // [group][channel][4 slices]: one ds_read_b128 per tap
-                // and group, all at immediate offsets from one address
-                const float4* q = reinterpret_cast<const float4*>(sview) + idx[k];
-#pragma unroll
-                for (int g = 0; g < NB / 4; ++g) {
-                    const float4 a = q[g * LROW], c = q[g * LROW + 1];
-                    p0[k][4 * g] = a.x;
-                    p0[k][4 * g + 1] = a.y;
-                    p0[k][4 * g + 2] = a.z;
-                    p0[k][4 * g + 3] = a.w;
-                    p1[k][4 * g] = c.x;
-                    p1[k][4 * g + 1] = c.y;
-                    p1[k][4 * g + 2] = c.z;
-                    p1[k][4 * g + 3] = c.w;
-                }
-            } else {
-                // Right taps come through a second base whose +1 the
-                // compiler cannot see, so the loads pair up per tap
-                // across slices (rows LROW words apart:
-                // ds_read2st64_b32, no per-slice address arithmetic).
-                int one = 1;
-                asm volatile("" : "+v"(one));
-                const float* q0 = sview + idx[k];
-                const float* q1 = sview + idx[k] + one;
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    p0[k][b] = q0[b * LROW];
-                    p1[k][b] = q1[b * LROW];
-                }
-            }
-        }
-        float cw0[PPT], cw1[PPT];
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            cw0[k] = w0[k];
-            cw1[k] = w1[k];
-        }
-        if (t + 1 < nv) {
-            const int inside = uniform_i(An.inside), is0 = uniform_i(An.is0);
-#pragma unroll
-            for (int k = 0; k < PPT; ++k)
-                tap_weights<Geo, LROW>(P, An, inside, is0, fi[k], fj[k], idx[k], w0[k],
-                                       w1[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < PPT; ++k)
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-                acc[k][b] = fmaf(cw1[k], p1[k][b], fmaf(cw0[k], p0[k][b], acc[k][b]));
+    // The anchor after a chunk's last view is read but not used: index
+    // vc + nv <= kAnchorChunk, at worst the first words of the span buffers.
+    const typename Geo::Anchor* an = anchors + vc;
+    for (int t = 0; t < nv; t += 2) {
+        bp_view<Geo, NB, LROW, PPT, EDGE>(Pg, an + t + 1, t + 1 < nv, sbuf + t * NB * LROW, fi,
+                                          fj, ta, tb, acc);
+        if (t + 1 < nv)
+            bp_view<Geo, NB, LROW, PPT, EDGE>(Pg, an + t + 2, t + 2 < nv,
+                                              sbuf + (t + 1) * NB * LROW, fi, fj, tb, ta, acc);
+    }
+}
+
+// The views of one anchor chunk [va, va + nva), staged chunk by chunk
+// through the two span buffers (the first chunk's loads already issued).
+template <class Geo, int NB, int LROW, int NWAVES, int PPT, bool EDGE>
+__device__ __forceinline__ void bp_anchor_chunk(const BpLaunch<Geo>& L,
+                                                const typename Geo::Problem& Pg,
+                                                const typename Geo::Anchor* anchors, float* spans,
+                                                int va, int nva, int b0, int nb, int wave,
+                                                int lane, const float (&fi)[PPT],
+                                                const float (&fj)[PPT], float (&acc)[PPT][NB])
+{
+    const int bufsz = L.vchunk * NB * LROW;  // floats per buffer
+    int cur = 0;
+    for (int vc = 0; vc < nva; vc += L.vchunk) {
+        const int nv = min(L.vchunk, nva - vc);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's span loads have landed
+        __syncthreads();                // ... and every other wave's
+        // prefetch the next chunk into the other buffer, accumulate this one
+        bp_chunk<Geo, NB, LROW, NWAVES, PPT, EDGE>(
+            L, Pg, anchors, spans + (cur ^ 1) * bufsz, spans + cur * bufsz, vc + L.vchunk < nva,
+            va, vc, nv, min(L.vchunk, nva - vc - L.vchunk), b0, nb, wave, lane, fi, fj, acc);
+        cur ^= 1;
     }
 }
 
@@ -328,7 +390,6 @@ __device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpL
                   "span rows are whole DMA rows (16 channels x 4 slices, or 64 channels)");
     Anchor* anchors = reinterpret_cast<Anchor*>(smem);
     float* spans = reinterpret_cast<float*>(smem + kAnchorChunk * sizeof(Anchor));
-    const int bufsz = L.vchunk * NB * LROW;  // floats per buffer
 
     const auto& P = L.P;
     const int tid = threadIdx.x, lane = tid & 63, wave = uniform_i(tid >> 6);
@@ -337,6 +398,8 @@ __device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpL
     const int ilo = F.x0 - F.ic, ihi = F.x0 + TW - 1 - F.ic;
     const int jlo = F.y0 - F.jc, jhi = F.y0 + TH - 1 - F.jc;
 
+    typename Geo::Problem Pg = P;
+    if constexpr (HasPin<Geo>::value) Geo::pin(Pg);
     float fi[PPT], fj[PPT];
     float total[PPT][NB];
 #pragma unroll
@@ -362,18 +425,14 @@ __device__ __forceinline__ void bp_tile(const typename Geo::Tables& T, const BpL
 #pragma unroll
             for (int b = 0; b < NB; ++b) acc[k][b] = 0.0f;
 
-        int cur = 0;
-        for (int vc = 0; vc < nva; vc += L.vchunk) {
-            const int nv = min(L.vchunk, nva - vc);
-            __builtin_amdgcn_s_waitcnt(0);  // this wave's span loads have landed
-            __syncthreads();                // ... and every other wave's
-            // prefetch the next chunk into the other buffer, accumulate this one
-            bp_chunk<Geo, NB, LROW, NWAVES, PPT>(
-                L, anchors, spans + (cur ^ 1) * bufsz, spans + cur * bufsz,
-                vc + L.vchunk < nva, va, vc, nv, min(L.vchunk, nva - vc - L.vchunk), b0, nb, wave,
-                lane, fi, fj, acc);
-            cur ^= 1;
-        }
+        // chunks whose views all keep the tile strictly inside the detector
+        // (most of them) skip the per-tap edge test
+        if (!HasCustomTaps<Geo>::value && __ballot(lane < nva && !anchors[lane].inside) != 0)
+            bp_anchor_chunk<Geo, NB, LROW, NWAVES, PPT, true>(L, Pg, anchors, spans, va, nva, b0,
+                                                              nb, wave, lane, fi, fj, acc);
+        else
+            bp_anchor_chunk<Geo, NB, LROW, NWAVES, PPT, false>(L, Pg, anchors, spans, va, nva,
+                                                               b0, nb, wave, lane, fi, fj, acc);
 #pragma unroll
         for (int k = 0; k < PPT; ++k)
 #pragma unroll

@@ -150,6 +150,11 @@ struct FanGeo {
     {
         fan_pixel<EXACT>(P, A, fi, fj, f, w);
     }
+    // the series' addends p1 and g1 meet the uniform p2 and g2 in one FMA each
+    __device__ __forceinline__ static void pin(Problem& P)
+    {
+        asm volatile("" : "+v"(P.p1), "+v"(P.g1));
+    }
 };
 
 }  // namespace ctr
