@@ -460,7 +460,8 @@ constexpr int kRows = 16, kRowPad = 16;
 __host__ __device__ __forceinline__ int row_stride(int ns) { return (kRowPad + ns + 66) | 1; }
 
 template <int NC, int TPW>
-__global__ __launch_bounds__(256) void ramp_rows_bf16x3_kernel(const ToeplitzParams P)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+ramp_rows_bf16x3_kernel(const ToeplitzParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int ns = P.ns;
@@ -532,49 +533,72 @@ __global__ __launch_bounds__(256) void ramp_rows_bf16x3_kernel(const ToeplitzPar
 #pragma unroll
     for (int t = 0; t < TPW; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    for (int i0 = 0; i0 < kdim; i0 += 32) {
-        // A[m][k]: input i = i0 + 8g + e of parity kpar in row m (zero past kdim)
-        const float* arow = xr + m * xs + kRowPad + 2 * (i0 + 8 * g) + kpar;
-        bf16x8 fa_hi, fa_lo;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float a = arow[2 * e];
-            const __bf16 h = static_cast<__bf16>(a);
-            fa_hi[e] = h;
-            fa_lo[e] = static_cast<__bf16>(a - static_cast<float>(h));
+    // B fragment of tile t at K step i0: B[k][n] = ho[q], q = j - i + pi - 1,
+    // j = 16 (tb0 + t) + n, i = i0 + 8g + e; taps Q, Q-1, ..., Q-7 are the
+    // reversed entries S .. S+7, S = nq-1-Q.  Every wave runs TPW tiles (at
+    // most one past the parity's last: its taps are still inside the
+    // copies, and its results are dropped).
+    auto load_b = [&](int t, int i0, bf16x8& fb_hi, bf16x8& fb_lo) {
+        const int Q = 16 * (tb0 + t) + m - (i0 + 8 * g) + pi - 1 - qmin;
+        const int S = nq - 1 - Q;
+        const int c = S & (NC - 1), bo = c * nrev + (S - c);
+        if constexpr (NC == 8) {
+            fb_hi = *reinterpret_cast<const bf16x8*>(bhi + bo);
+            fb_lo = *reinterpret_cast<const bf16x8*>(blo + bo);
+        } else {
+            const bf16x4 h0 = *reinterpret_cast<const bf16x4*>(bhi + bo);
+            const bf16x4 h1 = *reinterpret_cast<const bf16x4*>(bhi + bo + 4);
+            const bf16x4 l0 = *reinterpret_cast<const bf16x4*>(blo + bo);
+            const bf16x4 l1 = *reinterpret_cast<const bf16x4*>(blo + bo + 4);
+            fb_hi = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
+            fb_lo = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
         }
-        // every wave runs TPW tiles (at most one past the parity's last: its
-        // taps are still inside the copies, and its results are dropped)
+    };
+    // G is Toeplitz in 16 x 32 blocks: tile t's fragment at step i0 + 32 is
+    // tile t - 2's at step i0.  The fragments stay in a ring of TPW register
+    // slots (tile t of step s in slot (t - 2s) mod TPW); the K loop is
+    // unrolled over the ring's period TPW / 2 so every slot index is static,
+    // and a step loads only tiles 0 and 1 (their MFMAs come last in the step,
+    // behind the loads' latency).
+    constexpr int R = TPW / 2;
+    bf16x8 rh[TPW], rl[TPW];
 #pragma unroll
-        for (int t = 0; t < TPW; ++t) {
-            {
-            // B[k][n] = ho[q], q = j - i + pi - 1, j = 16 (tb0 + t) + n, i = i0 + 8g + e;
-            // taps Q, Q-1, ..., Q-7 = reversed entries S .. S+7, S = nq-1-Q
-            const int Q = 16 * (tb0 + t) + m - (i0 + 8 * g) + pi - 1 - qmin;
-            const int S = nq - 1 - Q;
-            const int c = S & (NC - 1), bo = c * nrev + (S - c);
-            bf16x8 fb_hi, fb_lo;
-            if constexpr (NC == 8) {
-                fb_hi = *reinterpret_cast<const bf16x8*>(bhi + bo);
-                fb_lo = *reinterpret_cast<const bf16x8*>(blo + bo);
-            } else {
-                const bf16x4 h0 = *reinterpret_cast<const bf16x4*>(bhi + bo);
-                const bf16x4 h1 = *reinterpret_cast<const bf16x4*>(bhi + bo + 4);
-                const bf16x4 l0 = *reinterpret_cast<const bf16x4*>(blo + bo);
-                const bf16x4 l1 = *reinterpret_cast<const bf16x4*>(blo + bo + 4);
-                fb_hi = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
-                fb_lo = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
-            }
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_lo, fb_hi, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi, fb_lo, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_hi, fb_hi, acc[t], 0, 0, 0);
+    for (int t = 2; t < TPW; ++t) load_b(t, 0, rh[t], rl[t]);
+    for (int base = 0; base < kdim; base += 32 * R) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const int i0 = base + 32 * s;
+            if (i0 < kdim) {  // uniform; once false, false for the rest
+                // A[m][k]: input i = i0 + 8g + e of parity kpar in row m (zero past kdim)
+                const float* arow = xr + m * xs + kRowPad + 2 * (i0 + 8 * g) + kpar;
+                bf16x8 fa_hi, fa_lo;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float a = arow[2 * e];
+                    const __bf16 h = static_cast<__bf16>(a);
+                    fa_hi[e] = h;
+                    fa_lo[e] = static_cast<__bf16>(a - static_cast<float>(h));
+                }
+                load_b(0, i0, rh[(TPW - 2 * s) % TPW], rl[(TPW - 2 * s) % TPW]);
+                load_b(1, i0, rh[(TPW + 1 - 2 * s) % TPW], rl[(TPW + 1 - 2 * s) % TPW]);
+#pragma unroll
+                for (int t = TPW - 1; t >= 0; --t) {
+                    const int sl = (TPW + t - 2 * s) % TPW;
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh[sl], fa_lo, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rl[sl], fa_hi, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh[sl], fa_hi, acc[t], 0, 0, 0);
+                }
             }
         }
     }
 
     // out[n] = ds (h0 xw[n] + sum_q h[2q+1] xw[n-1-2q] + far), n = 2j + pi,
-    // centre and |2q+1| <= 2 kNear - 1 in FP64; D[4g + u][m] is row 4g + u,
-    // column j = 16 (tb0 + t) + m.
+    // centre and |2q+1| <= 2 kNear - 1 in FP64.  The MFMAs take the taps as
+    // their first operand, so D[4g + u][m] is row m, column
+    // j = 16 (tb0 + t) + 4g + u: a lane's four outputs are consecutive
+    // channels of one parity in one row, and their near fields share one
+    // window of 2 kNear + 3 opposite-parity inputs (each read and widened
+    // once for the four).
     const double h0 = P.taps64[ns];
     double hn[2 * kNear];
 #pragma unroll
@@ -582,28 +606,34 @@ __global__ __launch_bounds__(256) void ramp_rows_bf16x3_kernel(const ToeplitzPar
         const int mm = 2 * q + 1;
         hn[q + kNear] = (mm >= -ns && mm <= ns) ? P.taps64[ns + mm] : 0.0;
     }
+    constexpr int kWinU = 2 * kNear + 3;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         if (t >= ntw) continue;
-        const int j = 16 * (tb0 + t) + m;
+        const int jb = 16 * (tb0 + t) + 4 * g;
+        // row[2u] is the input at n_u = 2 (jb + u) + pi; the near input
+        // n_u - 1 - 2q is row[2u - 1 - 2q] = win[u - q + kNear - 1]
+        const float* row = xr + m * xs + kRowPad + 2 * jb + pi;
+        double win[kWinU];
+#pragma unroll
+        for (int i = 0; i < kWinU; ++i) win[i] = static_cast<double>(row[2 * i + 1 - 2 * kNear]);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const float* row = xr + (4 * g + u) * xs + kRowPad + 2 * j + pi;
-            double near = h0 * static_cast<double>(row[0]);
+            double near = h0 * static_cast<double>(row[2 * u]);
 #pragma unroll
             for (int q = -kNear; q < kNear; ++q)
-                near = fma(hn[q + kNear], static_cast<double>(row[-1 - 2 * q]), near);
+                near = fma(hn[q + kNear], win[u - q + kNear - 1], near);
             acc[t][u] = static_cast<float>(P.ds * (near + static_cast<double>(acc[t][u])));
         }
     }
     __syncthreads();  // every wave has read the rows: they take the results
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-        const int j = 16 * (tb0 + t) + m;
-        if (t < ntw && j < nout) {
+        if (t >= ntw) continue;
+        const int jb = 16 * (tb0 + t) + 4 * g;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xr[(4 * g + u) * xs + kRowPad + 2 * j + pi] = acc[t][u];
-        }
+        for (int u = 0; u < 4; ++u)
+            if (jb + u < nout) xr[m * xs + kRowPad + 2 * (jb + u) + pi] = acc[t][u];
     }
     __syncthreads();
 #pragma unroll
