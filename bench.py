@@ -39,10 +39,11 @@ PEAK_VALU_SLOTS = CU * LANES * CLK  # FP32 VALU lane-issues per second (3.93e13)
 # Algorithmic FP32 VALU issue slots of the fan BP, counting a quarter-rate
 # transcendental (v_rcp_f32) as 4 slots.  Geometry, once per pixel*view and
 # shared by every slice the launch carries: numerator 2, denominator 2, rcp 4,
-# v = num*rcp 1, v^2 1, atan series 2, s = s_c + ... 2, weight 4, floor 1,
-# fraction 1, int convert 1, clamp 1, two tap weights 2, tap address 2 = 26.
+# v = num*rcp 1, v^2 1, atan series 2, s = s_c + ... 2, weight 4, fraction 1
+# (v_fract_f32; the int convert truncates, s >= 1 in the staged span), int
+# convert 1, clamp 1, two tap weights 2, tap address 2 = 25.
 # Per slice: two FMAs into the accumulator = 2.
-FAN_GEOM_SLOTS = 26
+FAN_GEOM_SLOTS = 25
 FAN_SLICE_SLOTS = 2
 # Per-slice model with no geometry sharing (atan2 + divide per pixel*view,
 # the textbook count for one slice at a time).

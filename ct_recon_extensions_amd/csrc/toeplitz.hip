@@ -302,6 +302,7 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const ToeplitzParams P)
 constexpr int kAS = kKC + 8;  // bf16 row stride of the A splits (144 B)
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // Entries per tap copy (a multiple of 8 so every copy starts 16-byte aligned).
 __host__ __device__ __forceinline__ int rev_len(int nq) { return ((nq + 7) & ~7) + 8; }
@@ -440,27 +441,38 @@ __global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P
     }
 }
 
-// Whole-row BF16x3 filter for rows of up to ~900 channels: one workgroup
+// Whole-row BF16x3 filter for rows of up to 1024 channels: one workgroup
 // filters kRows rows.  The rows are loaded once, fan-weighted, into LDS in
-// FP32 with kRowPad zeros on the left and at least 64 on the right:
+// FP32 with kRowPad zeros on the left and at least 16 on the right:
 //   xr[r][kRowPad + k] = w[k] x[r0 + r][k].
 // Wave w computes output parity pi = w >> 1 and half w & 1 of that parity's
 // 16-wide column tiles (TPW of them, the half rounded up to an even count):
-// the far field on the matrix cores (A fragments split
-// into BF16 hi / lo from the FP32 rows as they are read, B fragments from
-// the reversed tap copies, as in ramp_bf16x3_kernel), then the centre and
-// near taps in FP64 from the same rows.  The results go back into the row
-// buffer and leave as whole rows, so input and output move through HBM once
-// each, with contiguous accesses.
+// the far field on the matrix cores (row fragments split into BF16 hi / lo
+// from the FP32 rows as they are read; tap fragments from the reversed tap
+// vector, one word per entry holding its BF16 hi and lo, kept in a register
+// ring across K steps), then the centre and near taps in FP64 from the same
+// rows.  The results go back into the row buffer and leave as whole rows,
+// so input and output move through HBM once each, with contiguous accesses.
 constexpr int kRows = 16, kRowPad = 16;
 
-// Row stride (floats): kRowPad + ns + >= 66 zeros, odd, so that the 64
-// lanes of an A-fragment read (16 rows x 4 column groups 16 floats apart)
-// hit 64 different banks.
-__host__ __device__ __forceinline__ int row_stride(int ns) { return (kRowPad + ns + 66) | 1; }
+// Row stride (floats): kRowPad + ns + >= 16 zeros (the near field reaches
+// 15 channels past an output), odd, so that the 64 lanes of an A-fragment
+// read (16 rows x 4 column groups 16 floats apart) hit 64 different banks.
+__host__ __device__ __forceinline__ int row_stride(int ns) { return (kRowPad + ns + 16) | 1; }
 
-template <int NC, int TPW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+// LDS bytes of the whole-row kernel: the rows, then one word per entry of
+// the reversed far-field tap vector.
+__host__ __device__ __forceinline__ size_t rows_kernel_lds(int ns)
+{
+    return (static_cast<size_t>(kRows) * row_stride(ns) + rev_len(ns + kKC + kTN + 10)) *
+           sizeof(float);
+}
+
+// Up to 12 tiles per wave the kernel fits 170 VGPRs, so three workgroups
+// share a CU when their LDS allows (rows of up to 763 channels).
+template <int TPW>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(TPW <= 12 ? 3 : 2, TPW <= 12 ? 3 : 2))) void
 ramp_rows_bf16x3_kernel(const ToeplitzParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -469,23 +481,23 @@ ramp_rows_bf16x3_kernel(const ToeplitzParams P)
     const int nq = ns + kKC + kTN + 10;
     const int nrev = rev_len(nq);
     const int xs = row_stride(ns);
-    // LDS: B copies hi/lo [NC][nrev] bf16 | rows [kRows][xs] floats
-    __bf16* bhi = reinterpret_cast<__bf16*>(smem);
-    __bf16* blo = bhi + NC * nrev;
-    float* xr = reinterpret_cast<float*>(smem + ((static_cast<size_t>(4 * NC) * nrev + 15) &
-                                                 ~static_cast<size_t>(15)));
+    // LDS: rows [kRows][xs] floats | taps [nrev] words (bf16 hi | bf16 lo << 16).
+    // The taps follow the rows, so reads a little past the last row (by
+    // lanes whose results are dropped) stay inside the allocation.
+    float* xr = reinterpret_cast<float*>(smem);
+    unsigned int* bc = reinterpret_cast<unsigned int*>(xr + kRows * xs);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int r0 = blockIdx.x * kRows;
 
-    // the tap copies (the layout bf16_tap_copies_kernel writes for the tiled
-    // kernel), straight from the FP64 taps
-    for (int e = tid; e < NC * nrev; e += 256) {
-        const int c = e / nrev, j = e - c * nrev;
-        const float f = far_tap_entry(P.taps64, ns, nq, qmin, j + c);
+    // the reversed far-field taps, straight from the FP64 taps, split into
+    // BF16 hi and lo in one word
+    for (int e = tid; e < nrev; e += 256) {
+        const float f = far_tap_entry(P.taps64, ns, nq, qmin, e);
         const __bf16 h = static_cast<__bf16>(f);
-        bhi[e] = h;
-        blo[e] = static_cast<__bf16>(f - static_cast<float>(h));
+        const __bf16 l = static_cast<__bf16>(f - static_cast<float>(h));
+        bc[e] = static_cast<unsigned int>(__builtin_bit_cast(unsigned short, h)) |
+                (static_cast<unsigned int>(__builtin_bit_cast(unsigned short, l)) << 16);
     }
     // Rows in, weighted: channel k = tid + 256 c (ns <= 1024), eight rows'
     // loads in flight at a time; zeros in the margins.
@@ -540,19 +552,16 @@ ramp_rows_bf16x3_kernel(const ToeplitzParams P)
     // copies, and its results are dropped).
     auto load_b = [&](int t, int i0, bf16x8& fb_hi, bf16x8& fb_lo) {
         const int Q = 16 * (tb0 + t) + m - (i0 + 8 * g) + pi - 1 - qmin;
-        const int S = nq - 1 - Q;
-        const int c = S & (NC - 1), bo = c * nrev + (S - c);
-        if constexpr (NC == 8) {
-            fb_hi = *reinterpret_cast<const bf16x8*>(bhi + bo);
-            fb_lo = *reinterpret_cast<const bf16x8*>(blo + bo);
-        } else {
-            const bf16x4 h0 = *reinterpret_cast<const bf16x4*>(bhi + bo);
-            const bf16x4 h1 = *reinterpret_cast<const bf16x4*>(bhi + bo + 4);
-            const bf16x4 l0 = *reinterpret_cast<const bf16x4*>(blo + bo);
-            const bf16x4 l1 = *reinterpret_cast<const bf16x4*>(blo + bo + 4);
-            fb_hi = __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7);
-            fb_lo = __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7);
+        const unsigned int* w = bc + (nq - 1 - Q);
+        u32x4 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const unsigned int a = w[2 * e], b = w[2 * e + 1];
+            hi[e] = (a & 0xffffu) | (b << 16);
+            lo[e] = (a >> 16) | (b & 0xffff0000u);
         }
+        fb_hi = __builtin_bit_cast(bf16x8, hi);
+        fb_lo = __builtin_bit_cast(bf16x8, lo);
     };
     // G is Toeplitz in 16 x 32 blocks: tile t's fragment at step i0 + 32 is
     // tile t - 2's at step i0.  The fragments stay in a ring of TPW register
@@ -569,12 +578,13 @@ ramp_rows_bf16x3_kernel(const ToeplitzParams P)
         for (int s = 0; s < R; ++s) {
             const int i0 = base + 32 * s;
             if (i0 < kdim) {  // uniform; once false, false for the rest
-                // A[m][k]: input i = i0 + 8g + e of parity kpar in row m (zero past kdim)
+                // A[m][k]: input i = i0 + 8g + e of parity kpar in row m
                 const float* arow = xr + m * xs + kRowPad + 2 * (i0 + 8 * g) + kpar;
                 bf16x8 fa_hi, fa_lo;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const float a = arow[2 * e];
+                    // past kdim the row holds zeros for 16 channels, then the next row
+                    const float a = i0 + 8 * g + e < kdim ? arow[2 * e] : 0.0f;
                     const __bf16 h = static_cast<__bf16>(a);
                     fa_hi[e] = h;
                     fa_lo[e] = static_cast<__bf16>(a - static_cast<float>(h));
@@ -691,10 +701,9 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
     const int nc = bf16_copies();
     if (bf16x3) {
         // Whole rows in LDS while two workgroups fit per CU and a wave's tiles
-        // fit its accumulators: the kernel builds its own tap copies and
+        // fit its accumulators: the kernel builds its own tap vector and
         // weights (no scratch, no preparation launches).
-        const size_t rows_lds = ((static_cast<size_t>(4 * nc) * nrev + 15) & ~static_cast<size_t>(15)) +
-                                static_cast<size_t>(kRows) * row_stride(ns) * sizeof(float);
+        const size_t rows_lds = rows_kernel_lds(ns);
         const int n_even = (ns + 1) / 2;
         const int tper = (div_up(n_even, 16) + 1) / 2;
         if (rows_lds <= 80 * 1024 && tper <= 16 && ns <= 1024) {
@@ -711,14 +720,14 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
             P.dso = g.dso;
             P.dsd = g.dsd;
             P.cents = cent_s(g);
-            auto rk = tper <= 2    ? ramp_rows_bf16x3_kernel<kBf16Copies, 2>
-                      : tper <= 4  ? ramp_rows_bf16x3_kernel<kBf16Copies, 4>
-                      : tper <= 6  ? ramp_rows_bf16x3_kernel<kBf16Copies, 6>
-                      : tper <= 8  ? ramp_rows_bf16x3_kernel<kBf16Copies, 8>
-                      : tper <= 10 ? ramp_rows_bf16x3_kernel<kBf16Copies, 10>
-                      : tper <= 12 ? ramp_rows_bf16x3_kernel<kBf16Copies, 12>
-                      : tper <= 14 ? ramp_rows_bf16x3_kernel<kBf16Copies, 14>
-                                   : ramp_rows_bf16x3_kernel<kBf16Copies, 16>;
+            auto rk = tper <= 2    ? ramp_rows_bf16x3_kernel<2>
+                      : tper <= 4  ? ramp_rows_bf16x3_kernel<4>
+                      : tper <= 6  ? ramp_rows_bf16x3_kernel<6>
+                      : tper <= 8  ? ramp_rows_bf16x3_kernel<8>
+                      : tper <= 10 ? ramp_rows_bf16x3_kernel<10>
+                      : tper <= 12 ? ramp_rows_bf16x3_kernel<12>
+                      : tper <= 14 ? ramp_rows_bf16x3_kernel<14>
+                                   : ramp_rows_bf16x3_kernel<16>;
             if (rows_lds > 64 * 1024)
                 CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rk),
                                             hipFuncAttributeMaxDynamicSharedMemorySize,

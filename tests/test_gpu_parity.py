@@ -370,10 +370,11 @@ def test_autograd_fan_fbp_backward_is_transpose(F, det):
 
 
 # ------------------------------------------- ramp filter on the matrix cores
-# 37 rows: a partial last row block; 927 / 928 channels sit either side of
-# the whole-row BF16x3 kernel's LDS limit (longer rows take the tiled one).
+# 37 rows: a partial last row block.  The whole-row BF16x3 kernel takes rows
+# of up to 1024 channels (three workgroups per CU up to 763, two beyond);
+# 1025 and longer take the tiled one.
 @pytest.mark.parametrize("impl", ["mfma", "bf16x3"])
-@pytest.mark.parametrize("ns", [2, 3, 33, 64, 367, 736, 927, 928, 1472])
+@pytest.mark.parametrize("ns", [2, 3, 33, 64, 367, 736, 763, 764, 927, 1024, 1025, 1472])
 @pytest.mark.parametrize("typ,name", [(0, "par"), (1, "fan")])
 def test_ramp_matrix_cores_vs_oracle(F, ns, typ, name, impl):
     nrow = 37
