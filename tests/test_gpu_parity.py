@@ -120,6 +120,47 @@ def test_fan_bp_launch_configs_match_oracle(F, tile, vchunk, nb):
         assert rel_l2(out[b], O.fan_bp(g, proj[b], t)) < TOL32, b
 
 
+def _fan_chunk_kinds(geo, tables, tw=16, th=16, chunk=64):
+    """(all-inside, edge) counts of (pixel tile, 64-view chunk) pairs: a
+    chunk is an edge chunk when some view puts a pixel of the tile within a
+    quarter channel of the detector's ends or past them (the BP runs its
+    per-pixel edge test only in those chunks)."""
+    xpos, ypos, sc = tables
+    g = geo
+    cents = (g["ns"] - 1) / 2 + g["offset_s"]
+    sb, cb = sc[0::2][:, None, None], sc[1::2][:, None, None]
+    x, y = xpos[None, None, :], ypos[None, :, None]
+    d = g["dso"] + x * sb - y * cb
+    r = x * cb + y * sb
+    s = np.arctan2(r, d) * g["dsd"] / g["ds"] + cents       # [view][iy][ix]
+    inside = (s >= 0.25) & (s <= g["ns"] - 1.25)
+    counts = [0, 0]
+    for ty in range(0, g["ny"], th):
+        for tx in range(0, g["nx"], tw):
+            tile_in = inside[:, ty:ty + th, tx:tx + tw].all(axis=(1, 2))
+            for v0 in range(0, g["na"], chunk):
+                counts[0 if tile_in[v0:v0 + chunk].all() else 1] += 1
+    return counts
+
+
+# A detector that covers the centre tiles for every view and the outer tiles
+# for only some, over 100 views (anchor chunks of 64 and 36): both copies of
+# the BP's view loop (with and without the per-pixel edge test) run in one
+# launch, on the slice-interleaved path (16 slices) and the pair path (3).
+@pytest.mark.parametrize("nbatch", [16, 3])
+def test_fan_bp_edge_and_inside_chunks(F, nbatch):
+    geo = dict(ns=128, na=100, ds=1.0, orbit=2 * math.pi / 100, offset_s=3.3, orbit_start=0.1,
+               nx=64, ny=64, dx=1.0, dy=1.0, offset_x=0.0, offset_y=0.0, dso=100.0, dsd=200.0)
+    g = O.geom(**geo)
+    t = O.fan_prep(g)
+    n_in, n_edge = _fan_chunk_kinds(geo, t)
+    assert n_in >= 4 and n_edge >= 4, (n_in, n_edge)
+    proj = randn(900 + nbatch, (nbatch, g.na, g.ns))
+    out = host(F.fan_backprojection_2d(cuda(proj), **fan_kwargs(geo)))
+    for b in range(nbatch):
+        assert rel_l2(out[b], O.fan_bp(g, proj[b], t)) < TOL32, b
+
+
 def test_fan_bp_float64(F):
     geo, d = load_golden("fan_small")
     out = host(F.fan_backprojection_2d(cuda(d["proj"][None], torch.float64), **fan_kwargs(geo)))
