@@ -215,8 +215,14 @@ __device__ __forceinline__ void tap_weights(const typename Geo::Problem& P,
         if constexpr (EDGE) {
             const int inside = uniform_i(A.inside), is0 = uniform_i(A.is0);
             if (!inside) w = on_detector(is0 + idx, fr, P.ns) ? w : 0.0f;
+            // keeps the reads inside the span for any geometry
+            idx = min(max(idx, 0), LROW - 2);
         }
-        idx = min(max(idx, 0), LROW - 2);
+        // Without the edge test every anchor of the chunk is finite with
+        // inside = 1, and the tile's positions lie within ~1e-5 channel of
+        // [smin, smax]: f >= 1 - eps and f < nch - 2 + eps with
+        // nch = floor(smax) + 3 - is0 <= lmax <= LROW (the launchers' span
+        // bound), so 0 <= idx <= LROW - 2 needs no clamp.
         w1 = w * fr;
         w0 = w - w1;
     }
