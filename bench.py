@@ -41,9 +41,10 @@ PEAK_VALU_SLOTS = CU * LANES * CLK  # FP32 VALU lane-issues per second (3.93e13)
 # shared by every slice the launch carries: numerator 2, denominator 2, rcp 4,
 # v = num*rcp 1, v^2 1, atan series 2, s = s_c + ... 2, weight 4, fraction 1
 # (v_fract_f32; the int convert truncates, s >= 1 in the staged span), int
-# convert 1, clamp 1, two tap weights 2, tap address 2 = 25.
+# convert 1, two tap weights 2, tap address 2 = 24 (the index clamp and the
+# per-pixel edge test run only for tiles that reach past the detector).
 # Per slice: two FMAs into the accumulator = 2.
-FAN_GEOM_SLOTS = 25
+FAN_GEOM_SLOTS = 24
 FAN_SLICE_SLOTS = 2
 # Per-slice model with no geometry sharing (atan2 + divide per pixel*view,
 # the textbook count for one slice at a time).
