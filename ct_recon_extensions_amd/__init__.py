@@ -11,6 +11,7 @@ from .functional import (fan_backprojection_2d, fan_backprojection_2d_angle,  # 
                          parallel_backprojection_2d, parallel_backprojection_2d_grad,
                          parallel_projection_2d, parallel_projection_2d_grad, ramp_filter,
                          ramp_filter_grad)
+from .torch_ops import bppar2d_forward, bppar2d_prep_forward  # noqa: F401
 from .geometry import (FanBackprojection2DAngleParam, FanBackprojection2DParam,  # noqa: F401
                        FanWeightingParam, FilterParam, ParallelBackprojection2DParam,
                        ParallelProjection2DParam)
