@@ -328,6 +328,38 @@ def bench_filter_precision(torch, reps):
     return res
 
 
+def device_spec():
+    """The GPU as this run sees it: torch's device properties, and the CU
+    count and peak engine clock of the gfx950 agent in rocminfo (run as a
+    child process), beside the published constants the roofline uses."""
+    import subprocess
+
+    import torch
+
+    spec = {"constants": {"cus": CU, "clock_hz": CLK, "hbm_gb_s": PEAK_HBM_GBS}}
+    try:
+        pr = torch.cuda.get_device_properties(0)
+        spec.update(name=pr.name, arch=getattr(pr, "gcnArchName", ""),
+                    cus=pr.multi_processor_count, memory_bytes=pr.total_memory)
+    except Exception as e:  # noqa: BLE001 - reported, not fatal
+        spec["torch_error"] = str(e)
+    try:
+        text = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=60).stdout
+        for agent in text.split("Agent ")[1:]:
+            if "gfx950" not in agent:
+                continue
+            for line in agent.splitlines():
+                key, _, val = line.strip().partition(":")
+                if key == "Compute Unit":
+                    spec["rocminfo_cus"] = int(val)
+                elif key == "Max Clock Freq. (MHz)":
+                    spec["rocminfo_max_clock_mhz"] = int(val)
+            break
+    except Exception as e:  # noqa: BLE001
+        spec["rocminfo_error"] = str(e)
+    return spec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -536,6 +568,7 @@ def main():
         result["rel_l2_vs_reference_f32"] = e32
         result["rel_l2_vs_fp64"] = e64
 
+    result["device"] = device_spec()
     print(json.dumps(result))
 
 
