@@ -9,6 +9,7 @@ from .functional import (fan_backprojection_2d, fan_backprojection_2d_angle,  # 
                          fan_backprojection_2d_angle_grad, fan_backprojection_2d_grad,
                          fan_filter, fan_weighting, fan_weighting_grad,
                          parallel_backprojection_2d, parallel_backprojection_2d_grad,
+                         parallel_single_view_bp_2d,
                          parallel_projection_2d, parallel_projection_2d_grad, ramp_filter,
                          ramp_filter_grad)
 from .torch_ops import bppar2d_forward, bppar2d_prep_forward  # noqa: F401

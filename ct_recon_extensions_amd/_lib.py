@@ -66,6 +66,7 @@ _SIGS = {
     "ctr_plain_fan_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, _P]),
     "ctr_par_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
     "ctr_par_bp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P]),
+    "ctr_par_sv_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
     "ctr_fp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
     "ctr_fp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_uint, _P]),
     "ctr_ramp": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, _P]),

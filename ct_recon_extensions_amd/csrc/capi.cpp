@@ -23,6 +23,8 @@ int par_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch
            const double* xcos, const double* ysin, hipStream_t st);
 int par_bpt(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
             const double* xcos, const double* ysin, bool det, hipStream_t st);
+int par_svbp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
+             const double* xcos, const double* ysin, hipStream_t st);
 int fp_par(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
            const double* sc, const double* bo, const int* usex, hipStream_t st);
 int fp_par_t(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
@@ -140,6 +142,18 @@ int ctr_par_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nb
     return guarded("ctr_par_bp", [&] {
         return ctr::par_bp(*g, dtype, proj, img, nbatch, xcos, ysin,
                            static_cast<hipStream_t>(stream));
+    });
+}
+
+int ctr_par_sv_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                  const double* xcos, const double* ysin, void* stream)
+{
+    int rc = check_common(g, dtype, proj, img, nbatch, "ctr_par_sv_bp");
+    if (rc) return rc;
+    CTR_REQUIRE(xcos && ysin, CTR_ERR_ARG, "ctr_par_sv_bp: null table");
+    return guarded("ctr_par_sv_bp", [&] {
+        return ctr::par_svbp(*g, dtype, proj, img, nbatch, xcos, ysin,
+                             static_cast<hipStream_t>(stream));
     });
 }
 

@@ -1,0 +1,129 @@
+// Single-view parallel BP, ParallelSingleViewBp2DPixDriven<T>
+// (src/bp_par_2d_sv.cc:54-92): every (pixel, view) keeps its own value,
+//     img[slice][iy][ix][ia] = (1 - u) proj[ia][is1] + u proj[ia][is2],
+//     s = (xcos[ix][ia] + ysin[iy][ia]) / ds + cents, is1 = floor(s),
+//     is2 = ceil(s), u = s - is1, zero unless 0 <= s <= ns - 1,
+// with no sum over views and no scale; the tables are the parallel BP's.
+//
+// The output (nx ny na values per slice, 755 MB at 512^2 x 720 views) makes
+// this an HBM-write-bound kernel.  A workgroup owns kSvPix consecutive
+// pixels (flattened iy*nx + ix) of one slice and walks the views in chunks
+// of kSvViews: it stages x cos + y sin of its pixels for the chunk in LDS
+// (coalesced table reads: each pixel's views are contiguous), computes with
+// lanes over pixels (so the two taps of neighbouring pixels in one view row
+// come from neighbouring channels), and writes each pixel's chunk of views
+// back as one contiguous run.  The arithmetic is the reference's, in FP64
+// without contraction, rounded to T once: the result matches it bit for bit.
+#include "ctr_common.hpp"
+
+namespace ctr {
+
+namespace {
+
+constexpr int kSvPix = 64, kSvViews = 32, kSvPad = kSvViews + 1;
+
+struct SvbpProblem {
+    int ns, na, npix;
+    double ds, cents;
+    double top;  // ns - 1
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ proj,
+                                                       T* __restrict__ img,
+                                                       const double* __restrict__ xcos,
+                                                       const double* __restrict__ ysin,
+                                                       const SvbpProblem P, int nx)
+{
+#pragma clang fp contract(off)
+    __shared__ double pos[kSvPix][kSvPad];  // x cos + y sin, [pixel][view]
+    __shared__ T res[kSvPix][kSvPad];       // results, [pixel][view]
+    const int tid = threadIdx.x;
+    const int pix0 = blockIdx.x * kSvPix;
+    const int b = blockIdx.y;
+    const T* pr = proj + static_cast<size_t>(b) * P.na * P.ns;
+    T* out = img + (static_cast<size_t>(b) * P.npix + pix0) * P.na;
+    // staging and writing: thread -> (pixel tid >> 2, views 8 (tid & 3) .. +7)
+    const int sp = tid >> 2, sv = 8 * (tid & 3);
+    const int spix = pix0 + sp;
+    const bool s_ok = spix < P.npix;
+    const int siy = s_ok ? spix / nx : 0, six = s_ok ? spix - siy * nx : 0;
+    const double* xc = xcos + static_cast<size_t>(six) * P.na;
+    const double* ys = ysin + static_cast<size_t>(siy) * P.na;
+    // computing: thread -> (pixel tid & 63, views 8 (tid >> 6) .. +7)
+    const int cp = tid & (kSvPix - 1), cv = 8 * (tid >> 6);
+    const bool c_ok = pix0 + cp < P.npix;
+
+    for (int va = 0; va < P.na; va += kSvViews) {
+        const int nv = min(kSvViews, P.na - va);
+        __syncthreads();  // the previous chunk's results are written out
+        if (s_ok) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int v = sv + j;
+                if (v < nv) pos[sp][v] = xc[va + v] + ys[va + v];
+            }
+        }
+        __syncthreads();
+        if (c_ok) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int v = cv + j;
+                if (v >= nv) break;
+                const double s = pos[cp][v] / P.ds + P.cents;
+                T r = T(0);
+                if (s >= 0 && s <= P.top) {
+                    const double f = floor(s);
+                    const int is1 = static_cast<int>(f), is2 = static_cast<int>(ceil(s));
+                    const double u = s - f;
+                    const T* row = pr + static_cast<size_t>(va + v) * P.ns;
+                    r = static_cast<T>((1 - u) * static_cast<double>(row[is1]) +
+                                       u * static_cast<double>(row[is2]));
+                }
+                res[cp][v] = r;
+            }
+        }
+        __syncthreads();
+        if (s_ok) {
+            T* o = out + static_cast<size_t>(sp) * P.na + va;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int v = sv + j;
+                if (v < nv) o[v] = res[sp][v];
+            }
+        }
+    }
+}
+
+}  // namespace
+
+int par_svbp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
+             const double* xcos, const double* ysin, hipStream_t st)
+{
+    CTR_REQUIRE(g.ns >= 1 && g.na >= 1 && g.nx >= 1 && g.ny >= 1, CTR_ERR_ARG,
+                "single-view BP: sizes must be positive");
+    CTR_REQUIRE(g.ds > 0, CTR_ERR_ARG, "single-view BP: ds must be positive");
+    CTR_REQUIRE(nbatch <= 65535, CTR_ERR_ARG, "single-view BP: at most 65535 slices per call");
+    const long long npix = static_cast<long long>(g.nx) * g.ny;
+    CTR_REQUIRE(npix < (1LL << 31) && static_cast<long long>(g.na) * g.ns < (1LL << 31),
+                CTR_ERR_SHAPE, "single-view BP: image or sinogram too large");
+    SvbpProblem P{};
+    P.ns = static_cast<int>(g.ns);
+    P.na = static_cast<int>(g.na);
+    P.npix = static_cast<int>(npix);
+    P.ds = g.ds;
+    P.cents = cent_s(g);
+    P.top = static_cast<double>(g.ns) - 1;
+    const dim3 grid(static_cast<unsigned int>((npix + kSvPix - 1) / kSvPix), nbatch);
+    if (dtype == CTR_F64)
+        hipLaunchKernelGGL(par_svbp_kernel<double>, grid, dim3(256), 0, st,
+                           static_cast<const double*>(proj), static_cast<double*>(img), xcos,
+                           ysin, P, static_cast<int>(g.nx));
+    else
+        hipLaunchKernelGGL(par_svbp_kernel<float>, grid, dim3(256), 0, st,
+                           static_cast<const float*>(proj), static_cast<float*>(img), xcos, ysin,
+                           P, static_cast<int>(g.nx));
+    return check_launch("par_svbp_kernel");
+}
+
+}  // namespace ctr

@@ -278,6 +278,27 @@ def parallel_backprojection_2d_grad(grad_in, img_shape, img_space, proj_shape, c
     return restore(_ParBPT.apply(x, p, tables, deterministic))
 
 
+def parallel_single_view_bp_2d(proj, img_shape, img_space, proj_shape, channel_space,
+                               img_offset=(0, 0), channel_offset=0, orbit_start=0,
+                               orbit=math.pi / 180, fov=-1, method="pixdriven"):
+    """Single-view parallel BP (python/tf_ctrecon.py:198-219;
+    src/bp_par_2d_sv.cc:54-92): every pixel keeps each view's interpolated
+    value, no sum and no scale.  proj [N, na, ns, 1] or [N, na, ns] -> img
+    [N, ny, nx, na]; [na, ns] -> [ny, nx, na].  Bit-identical to the
+    reference's CPU path.  The reference defines no gradient for it, and
+    neither does this op (the output does not require grad)."""
+    _check_method(method, ("pixdriven",), "parallel_single_view_bp_2d")
+    p = _par_param(img_shape, img_space, proj_shape, channel_space, img_offset, channel_offset,
+                   orbit_start, orbit, fov)
+    x, _ = _to_batch(proj, p.na, p.ns, "proj")
+    tables = device_tables("par", p, x.device)
+    n = x.shape[0]
+    out = torch.empty((n, p.ny, p.nx, p.na), dtype=x.dtype, device=x.device)
+    _call_batched("ctr_par_sv_bp", n, C.byref(p.ctr()), _dtype_code(x), _ptr(x), _ptr(out), n,
+                  _ptr(tables[0]), _ptr(tables[1]), _stream(x))
+    return out[0] if proj.dim() == 2 else out
+
+
 class _FP(torch.autograd.Function):
     @staticmethod
     def forward(ctx, img3, p, t_fp, t_grad):

@@ -123,6 +123,16 @@ int ctr_fan_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int 
  * (src/bp_par_2d.cc:51-90): img = orbit * sum_views lerp(proj, s). */
 int ctr_par_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
                const double* xcos, const double* ysin, void* stream);
+/* Single-view parallel BP, ParallelSingleViewBp2DPixDriven<T>
+ * (src/bp_par_2d_sv.cc:54-92; TF op SingleViewBpParallel2D,
+ * tensorflow/bp_par_2d_sv_ops.cu:46-85): proj [nbatch][na][ns] ->
+ * img [nbatch][ny][nx][na], each (pixel, view) the view's linear
+ * interpolation at s (zero off the detector), no sum and no scale; the
+ * tables are ctr_par_tables'.  Bit-identical to the reference's CPU path.
+ * The reference's TF launcher steps the output by nx*ny per slice; here it
+ * is nx*ny*na, so slices do not overlap. */
+int ctr_par_sv_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                  const double* xcos, const double* ysin, void* stream);
 /* Exact transpose of ctr_par_bp (replaces the reference's broken
  * ParallelBackprojection2DPixDrivenGrad<T>, src/bp_par_2d.cc:147-199). */
 int ctr_par_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,

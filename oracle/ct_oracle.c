@@ -231,6 +231,41 @@ int orc_par_bp_##SUF(const orc_geom* g, const T* proj, T* img, const double* xco
 DEF_PAR_BP(float, f)
 DEF_PAR_BP(double, d)
 
+/* Single-view parallel BP, src/bp_par_2d_sv.cc:54-92: every (pixel, view)
+ * keeps its own value, img[(iy*nx + ix)*na + ia] = the view's linear
+ * interpolation at s (zero off the detector), with no sum over views and no
+ * scale.  Its tables are the parallel BP's: src/bp_par_2d_sv.cc:18-49
+ * repeats src/bp_par_2d.cc:18-49 (orc_par_prep). */
+#define DEF_PAR_SVBP(T, SUF)                                                      \
+int orc_par_svbp_##SUF(const orc_geom* g, const T* proj, T* img,                  \
+                       const double* xcos, const double* ysin)                    \
+{                                                                                 \
+    const double cents = ((double)(g->ns - 1)) / 2 + g->offset_s;                 \
+    unsigned int ix, iy, ia, is1, is2;                                            \
+    for (iy = 0; iy < g->ny; ++iy) {                                              \
+        for (ix = 0; ix < g->nx; ++ix) {                                          \
+            T* out = img + ((size_t)iy * g->nx + ix) * g->na;                     \
+            for (ia = 0; ia < g->na; ++ia) {                                      \
+                const double s = (xcos[ia + ix * g->na] + ysin[ia + iy * g->na])  \
+                                 / g->ds + cents;                                 \
+                if (s >= 0 && s <= g->ns - 1) {                                   \
+                    double u;                                                     \
+                    is1 = (unsigned int)floor(s);                                 \
+                    is2 = (unsigned int)ceil(s);                                  \
+                    u = s - is1;                                                  \
+                    out[ia] = (T)((1 - u) * proj[is1 + ia * g->ns]                \
+                                  + u * proj[is2 + ia * g->ns]);                  \
+                } else {                                                          \
+                    out[ia] = 0;                                                  \
+                }                                                                 \
+            }                                                                     \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_PAR_SVBP(float, f)
+DEF_PAR_SVBP(double, d)
+
 /* Transpose of the parallel BP (scatter form, FP64 accumulation).  The
  * reference's transpose (src/bp_par_2d.cc:95-199) reads uninitialised
  * pointers, so this one is the mathematically intended operator, pinned by

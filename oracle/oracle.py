@@ -157,6 +157,23 @@ def par_bp(g: Geom, proj: np.ndarray, tables, kind="oracle"):
     return img
 
 
+def par_svbp_prep(g: Geom, kind="oracle"):
+    """Tables of the single-view parallel BP (src/bp_par_2d_sv.cc:18-49, the
+    same as the parallel BP's; the restatement reuses orc_par_prep)."""
+    xcos = np.zeros(g.na * g.nx); ysin = np.zeros(g.na * g.ny)
+    lib(kind).call("par_svbp_prep" if kind == "ref" else "par_prep", g, xcos, ysin)
+    return xcos, ysin
+
+
+def par_svbp(g: Geom, proj: np.ndarray, tables, kind="oracle"):
+    """Single-view parallel BP (src/bp_par_2d_sv.cc:54-92): img[iy, ix, ia]."""
+    proj = np.ascontiguousarray(proj)
+    img = np.zeros((g.ny, g.nx, g.na), proj.dtype)
+    xcos, ysin = tables
+    lib(kind).call("par_svbp_" + _suffix(proj.dtype), g, proj, img, xcos, ysin)
+    return img
+
+
 def par_bpt(g: Geom, img: np.ndarray, tables):
     img = np.ascontiguousarray(img)
     grad = np.zeros((g.na, g.ns), img.dtype)

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
 // the unchanged reference sources src/{bp_fan_2d,bp_fan_2d_angle,bp_par_2d,
-// fp_par_2d,filter,fan_weighting}.cc, read where they lie under the reference
+// bp_par_2d_sv,fp_par_2d,filter,fan_weighting}.cc, read where they lie under the reference
 // checkout, into oracle/_ref/libctref.so.  Nothing of the reference is copied
 // into this repository; the built library is git-ignored.  It is used to
 // write golden vectors (tests/golden/make_golden.py), to cross-check the C
@@ -14,6 +14,7 @@
 #include "include/bp_fan_2d.h"
 #include "include/bp_fan_2d_angle.h"
 #include "include/bp_par_2d.h"
+#include "include/bp_par_2d_sv.h"
 #include "include/fp_par_2d.h"
 #include "include/filter.h"
 #include "include/fan_weighting.h"
@@ -132,6 +133,34 @@ int ref_par_bp_d(const orc_geom* g, const double* p, double* img, const double* 
                  const double* ysin)
 {
     return ParallelBackprojection2DPixDriven<double>(par_param(g))
+        .calculate_on_cpu(p, img, xcos, ysin, nullptr);
+}
+
+static ParallelSingleViewBp2DParam svbp_param(const orc_geom* g)
+{
+    return ParallelSingleViewBp2DParam(g->ns, g->na, g->ds, g->orbit, g->offset_s,
+                                       g->orbit_start, g->nx, g->ny, g->dx, g->dy,
+                                       g->offset_x, g->offset_y, g->fov);
+}
+
+int ref_par_svbp_prep(const orc_geom* g, double* xcos, double* ysin)
+{
+    int unused = 0;
+    return ParallelSingleViewBp2DPixDrivenPrep(svbp_param(g)).calculate_on_cpu(xcos, ysin,
+                                                                               &unused);
+}
+
+int ref_par_svbp_f(const orc_geom* g, const float* p, float* img, const double* xcos,
+                   const double* ysin)
+{
+    return ParallelSingleViewBp2DPixDriven<float>(svbp_param(g))
+        .calculate_on_cpu(p, img, xcos, ysin, nullptr);
+}
+
+int ref_par_svbp_d(const orc_geom* g, const double* p, double* img, const double* xcos,
+                   const double* ysin)
+{
+    return ParallelSingleViewBp2DPixDriven<double>(svbp_param(g))
         .calculate_on_cpu(p, img, xcos, ysin, nullptr);
 }
 

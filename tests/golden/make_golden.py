@@ -56,6 +56,11 @@ FAN_C2_COARSE = dict(ns=736, na=720, ds=1.30, orbit=2 * math.pi / 720, offset_s=
 PAR_SMALL = dict(ns=41, na=36, ds=0.8, orbit=math.pi / 36, offset_s=-0.35,
                  orbit_start=0.1, nx=22, ny=26, dx=0.7, dy=0.9, offset_x=-0.4,
                  offset_y=0.25)
+# Single-view parallel BP: a detector narrower than the image, so that
+# samples fall past both of its ends (zeros in the output).
+SVBP_SMALL = dict(ns=15, na=12, ds=0.8, orbit=math.pi / 12, offset_s=0.4,
+                  orbit_start=0.2, nx=18, ny=14, dx=0.7, dy=0.9, offset_x=0.3,
+                  offset_y=-0.2)
 # Ray-driven FP: nx == ny (the reference's transpose requires it); the
 # detector covers the image so the reference's transpose stays defined.
 FP_SMALL = dict(ns=41, na=40, ds=1.0, orbit=math.pi / 40, offset_s=0.3,
@@ -110,6 +115,15 @@ def par_fixtures():
          bp_f64=O.par_bp(g, proj.astype(np.float64), t, kind="ref"))
 
 
+def svbp_fixtures():
+    g = O.geom(**SVBP_SMALL)
+    t = O.par_svbp_prep(g, "ref")
+    proj = randn(71, (g.na, g.ns))
+    save("svbp_small", g, proj=proj, xcos=t[0], ysin=t[1],
+         svbp_f32=O.par_svbp(g, proj, t, kind="ref"),
+         svbp_f64=O.par_svbp(g, proj.astype(np.float64), t, kind="ref"))
+
+
 def fp_fixtures():
     g = O.geom(**FP_SMALL)
     t = O.fp_prep(g, "ref")
@@ -161,5 +175,6 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/libctref.so missing: run `make -C oracle ref` first")
     fan_fixtures()
     par_fixtures()
+    svbp_fixtures()
     fp_fixtures()
     filter_fixtures()

@@ -62,6 +62,17 @@ def test_par_small():
     np.testing.assert_array_equal(O.par_bp(g, d["proj"].astype(np.float64), t), d["bp_f64"])
 
 
+def test_svbp_small():
+    """Single-view parallel BP against the compiled reference's outputs."""
+    geo, d = load_golden("svbp_small")
+    g = G(geo)
+    t = O.par_svbp_prep(g)
+    np.testing.assert_array_equal(t[0], d["xcos"])
+    np.testing.assert_array_equal(t[1], d["ysin"])
+    np.testing.assert_array_equal(O.par_svbp(g, d["proj"], t), d["svbp_f32"])
+    np.testing.assert_array_equal(O.par_svbp(g, d["proj"].astype(np.float64), t), d["svbp_f64"])
+
+
 def test_fp_small():
     geo, d = load_golden("fp_small")
     g = G(geo)

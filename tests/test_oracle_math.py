@@ -75,6 +75,38 @@ def test_par_bp_constant_sinogram():
     np.testing.assert_allclose(img, c * g.na * g.orbit, rtol=1e-13)
 
 
+def test_svbp_constant_and_detector_ends():
+    """Single-view BP of a constant c: c wherever the pixel's channel
+    position s = x cos + y sin (over ds, plus the centre) is on the detector,
+    0 elsewhere, with s computed here from the geometry."""
+    g = O.geom(ns=15, na=12, ds=0.8, orbit=math.pi / 12, offset_s=0.4, orbit_start=0.2,
+               nx=18, ny=14, dx=0.7, dy=0.9, offset_x=0.3, offset_y=-0.2)
+    c = 1.25
+    img = O.par_svbp(g, np.full((g.na, g.ns), c), O.par_prep(g))
+    x = (np.arange(g.nx) - ((g.nx - 1) / 2 + g.offset_x)) * g.dx
+    y = ((g.ny - 1) / 2 + g.offset_y - np.arange(g.ny)) * g.dy
+    th = g.orbit_start + g.orbit * np.arange(g.na)
+    s = (x[None, :, None] * np.cos(th) + y[:, None, None] * np.sin(th)) / g.ds \
+        + (g.ns - 1) / 2 + g.offset_s
+    on = (s >= 1e-9) & (s <= g.ns - 1 - 1e-9)
+    off = (s < -1e-9) | (s > g.ns - 1 + 1e-9)
+    assert on.sum() > 100 and off.sum() > 100
+    np.testing.assert_array_equal(img[on], c)
+    np.testing.assert_array_equal(img[off], 0)
+
+
+def test_svbp_views_sum_to_parallel_bp():
+    """orbit * sum over views of the single-view BP is the parallel BP (the
+    same per-view terms; the parallel BP adds them in FP64)."""
+    g = O.geom(ns=29, na=20, ds=0.9, orbit=math.pi / 20, offset_s=-0.3, orbit_start=0.05,
+               nx=16, ny=12, dx=0.8, dy=1.1, offset_x=0.1, offset_y=0.2)
+    t = O.par_prep(g)
+    p = np.random.default_rng(8).standard_normal((g.na, g.ns))
+    sv = O.par_svbp(g, p, t)
+    np.testing.assert_allclose(g.orbit * sv.sum(axis=-1), O.par_bp(g, p, t), rtol=1e-12,
+                               atol=1e-13)
+
+
 def test_fan_bp_constant_sinogram_matches_source_distances():
     """BP of a constant c is c*scale*sum_beta dsd^2/|P - S(beta)|^2, with the
     source at S = (-dso sin b, dso cos b) computed here independently."""
