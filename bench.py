@@ -152,6 +152,11 @@ def profile_kernels(torch, F, which):
         x = torch.randn((8, scan.n, scan.n), generator=g, device="cuda")
         for _ in range(3):
             img = fpt(fp(x))
+    elif which == "svbp":
+        c = SVBP_CFG
+        proj = torch.randn((1, c["na"], c["ns"]), generator=g, device="cuda")
+        for _ in range(3):
+            img = F.parallel_single_view_bp_2d(proj, **kw_svbp())
     else:
         proj = torch.randn((cfg["nbatch"], cfg["na"], cfg["ns"]), generator=g, device="cuda")
         for _ in range(3):
@@ -169,6 +174,47 @@ def profile_kernels(torch, F, which):
 
 
 C1_SEED = 20261206
+SVBP_SEED = 20261207
+# Single-view parallel BP workload: one 512^2 slice, parallel beam, 720 views
+# over 180 deg, 729 channels (ds = dx = dy = 1): 755 MB of output per launch.
+SVBP_CFG = dict(n=512, na=720, ns=729)
+
+
+def kw_svbp():
+    c = SVBP_CFG
+    return dict(img_shape=[c["n"], c["n"]], img_space=[1.0, 1.0], proj_shape=[c["na"], c["ns"]],
+                channel_space=1.0, orbit=math.pi / c["na"])
+
+
+def bench_svbp(torch, F, reps, with_cpu=True):
+    """Single-view parallel BP (src/bp_par_2d_sv.cc:54-92) of one seeded
+    random sinogram: GPU time per launch against the HBM roofline (the output,
+    nx*ny*na floats, dominates the algorithmic bytes), and the reference CPU
+    path on one thread for the same slice."""
+    c = SVBP_CFG
+    g = torch.Generator(device="cuda").manual_seed(SVBP_SEED)
+    proj = torch.randn((1, c["na"], c["ns"]), generator=g, device="cuda")
+    out = F.parallel_single_view_bp_2d(proj, **kw_svbp())
+    torch.cuda.synchronize()
+    ms = time_events(lambda: F.parallel_single_view_bp_2d(proj, **kw_svbp()), reps, torch)
+    n, na, ns = c["n"], c["na"], c["ns"]
+    alg = n * n * na * 4 + na * ns * 4 + 2 * n * na * 8
+    res = {"workload": "1 x 512^2, parallel 720 views/180deg, 729 ch, ds=dx=1; seeded N(0,1) "
+                       "sinogram", "seed": SVBP_SEED, "gpu_ms": round(ms, 4),
+           "algorithmic_bytes": alg, "gb_s": round(alg / (ms * 1e-3) / 1e9, 1),
+           "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    if with_cpu:
+        from oracle import oracle as O
+        kind = "ref" if O.have_ref() else "oracle"
+        geo = O.geom(ns=ns, na=na, ds=1.0, orbit=math.pi / na, nx=n, ny=n, dx=1.0, dy=1.0)
+        host = proj[0].cpu().numpy()
+        t = O.par_svbp_prep(geo, kind=kind)
+        t0 = time.perf_counter()
+        ref = O.par_svbp(geo, host, t, kind=kind)
+        cpu_s = time.perf_counter() - t0
+        res.update(cpu_kind="reference" if kind == "ref" else "port", cpu_1thread_ms=round(
+            cpu_s * 1e3, 1), bit_identical=bool(np.array_equal(out[0].cpu().numpy(), ref)))
+    return res
 
 
 def bench_c1(torch, F, threads, with_cpu=True):
@@ -374,14 +420,16 @@ def main():
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batch")
     ap.add_argument("--no-precision", action="store_true", help="skip the filter precision study")
     ap.add_argument("--no-c1", action="store_true", help="skip the 256^2 parallel FBP")
+    ap.add_argument("--no-svbp", action="store_true", help="skip the single-view parallel BP")
     ap.add_argument("--profile-bp", action="store_true",
                     help="counter-collection mode: three fan BP launches of the workload, then a "
                          "256 MiB device copy from a cold source (calibrates FETCH/WRITE_SIZE)")
-    ap.add_argument("--profile", choices=("bp", "filter", "lsq"), default=None,
+    ap.add_argument("--profile", choices=("bp", "filter", "lsq", "svbp"), default=None,
                     help="counter-collection mode for one kernel family: 'bp' as --profile-bp; "
                          "'filter' three fused fan-weighted ramp filters (default implementation) of the "
                          "workload; 'lsq' three FP and FP^T launches of the least-squares "
-                         "workload; each followed by the calibration copy")
+                         "workload; 'svbp' three single-view parallel BPs of its workload; each "
+                         "followed by the calibration copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path shards by slice with no exchange, so "
@@ -539,6 +587,11 @@ def main():
         result["c1_parallel_fbp"] = bench_c1(torch, F, args.cpu_threads or min(16, os.cpu_count() or 1),
                                              with_cpu=not args.no_cpu)
         print(f"[bench] c1 parallel FBP {result['c1_parallel_fbp']['gpu_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
+    if not args.no_svbp:
+        result["single_view_bp"] = bench_svbp(torch, F, 20, with_cpu=not args.no_cpu)
+        sv = result["single_view_bp"]
+        print(f"[bench] single-view BP {sv['gpu_ms']:.3f} ms, {sv['gb_s']:.0f} GB/s",
               file=sys.stderr, flush=True)
     if not args.no_precision:
         result["filter_precision"] = bench_filter_precision(torch, args.steps)

@@ -9,11 +9,13 @@
 // this an HBM-write-bound kernel.  A workgroup owns kSvPix consecutive
 // pixels (flattened iy*nx + ix) of one slice and walks the views in chunks
 // of kSvViews: it stages x cos + y sin of its pixels for the chunk in LDS
-// (coalesced table reads: each pixel's views are contiguous), computes with
-// lanes over pixels (so the two taps of neighbouring pixels in one view row
-// come from neighbouring channels), and writes each pixel's chunk of views
-// back as one contiguous run.  The arithmetic is the reference's, in FP64
-// without contraction, rounded to T once: the result matches it bit for bit.
+// (16-byte table reads, consecutive lanes on consecutive pieces of a
+// pixel's contiguous run of views), computes with lanes over pixels (so the
+// two taps of neighbouring pixels in one view row come from neighbouring
+// channels), and writes each pixel's chunk of views back with 16-byte
+// stores, consecutive lanes on consecutive pieces.  The arithmetic is the
+// reference's, in FP64 without contraction, rounded to T once: the result
+// matches it bit for bit.
 #include "ctr_common.hpp"
 
 namespace ctr {
@@ -28,14 +30,18 @@ struct SvbpProblem {
     double top;  // ns - 1
 };
 
+// 16-byte accesses of a pixel's run of views: VEC when the per-pixel strides
+// (na doubles of table, na values of output) and the buffers are 16-byte
+// aligned; otherwise element by element.
 template <typename T>
 __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ proj,
                                                        T* __restrict__ img,
                                                        const double* __restrict__ xcos,
                                                        const double* __restrict__ ysin,
-                                                       const SvbpProblem P, int nx)
+                                                       const SvbpProblem P, int nx, int vec)
 {
 #pragma clang fp contract(off)
+    constexpr int kTv = 16 / sizeof(T);        // output values per 16 bytes
     __shared__ double pos[kSvPix][kSvPad];  // x cos + y sin, [pixel][view]
     __shared__ T res[kSvPix][kSvPad];       // results, [pixel][view]
     const int tid = threadIdx.x;
@@ -43,13 +49,19 @@ __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ pro
     const int b = blockIdx.y;
     const T* pr = proj + static_cast<size_t>(b) * P.na * P.ns;
     T* out = img + (static_cast<size_t>(b) * P.npix + pix0) * P.na;
-    // staging and writing: thread -> (pixel tid >> 2, views 8 (tid & 3) .. +7)
-    const int sp = tid >> 2, sv = 8 * (tid & 3);
-    const int spix = pix0 + sp;
-    const bool s_ok = spix < P.npix;
-    const int siy = s_ok ? spix / nx : 0, six = s_ok ? spix - siy * nx : 0;
-    const double* xc = xcos + static_cast<size_t>(six) * P.na;
-    const double* ys = ysin + static_cast<size_t>(siy) * P.na;
+    // staging: element e = tid + 256 k (k < 4) of [64 pixels][16 pairs of
+    // views]: pixel e >> 4, views 2 (e & 15) and 2 (e & 15) + 1
+    const double* xc[4];
+    const double* ys[4];
+    bool s_ok[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int pix = pix0 + ((tid + 256 * k) >> 4);
+        s_ok[k] = pix < P.npix;
+        const int iy = s_ok[k] ? pix / nx : 0, ix = s_ok[k] ? pix - iy * nx : 0;
+        xc[k] = xcos + static_cast<size_t>(ix) * P.na;
+        ys[k] = ysin + static_cast<size_t>(iy) * P.na;
+    }
     // computing: thread -> (pixel tid & 63, views 8 (tid >> 6) .. +7)
     const int cp = tid & (kSvPix - 1), cv = 8 * (tid >> 6);
     const bool c_ok = pix0 + cp < P.npix;
@@ -57,11 +69,18 @@ __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ pro
     for (int va = 0; va < P.na; va += kSvViews) {
         const int nv = min(kSvViews, P.na - va);
         __syncthreads();  // the previous chunk's results are written out
-        if (s_ok) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int v = sv + j;
-                if (v < nv) pos[sp][v] = xc[va + v] + ys[va + v];
+        for (int k = 0; k < 4; ++k) {
+            const int e = tid + 256 * k, sp = e >> 4, v = 2 * (e & 15);
+            if (!s_ok[k] || v >= nv) continue;
+            if (vec && v + 1 < nv) {
+                const double2 x2 = *reinterpret_cast<const double2*>(xc[k] + va + v);
+                const double2 y2 = *reinterpret_cast<const double2*>(ys[k] + va + v);
+                pos[sp][v] = x2.x + y2.x;
+                pos[sp][v + 1] = x2.y + y2.y;
+            } else {
+                pos[sp][v] = xc[k][va + v] + ys[k][va + v];
+                if (v + 1 < nv) pos[sp][v + 1] = xc[k][va + v + 1] + ys[k][va + v + 1];
             }
         }
         __syncthreads();
@@ -84,12 +103,21 @@ __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ pro
             }
         }
         __syncthreads();
-        if (s_ok) {
-            T* o = out + static_cast<size_t>(sp) * P.na + va;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int v = sv + j;
-                if (v < nv) o[v] = res[sp][v];
+        // writing: element e of [64 pixels][32 / kTv pieces of 16 bytes]
+        constexpr int kPieces = kSvViews / kTv;
+        for (int e = tid; e < kSvPix * kPieces; e += 256) {
+            const int wp = e / kPieces, v = kTv * (e - wp * kPieces);
+            if (pix0 + wp >= P.npix || v >= nv) continue;
+            T* o = out + static_cast<size_t>(wp) * P.na + va + v;
+            if (vec && v + kTv <= nv) {
+                if constexpr (sizeof(T) == 4) {
+                    *reinterpret_cast<float4*>(o) =
+                        make_float4(res[wp][v], res[wp][v + 1], res[wp][v + 2], res[wp][v + 3]);
+                } else {
+                    *reinterpret_cast<double2*>(o) = make_double2(res[wp][v], res[wp][v + 1]);
+                }
+            } else {
+                for (int j = 0; j < kTv && v + j < nv; ++j) o[j] = res[wp][v + j];
             }
         }
     }
@@ -115,14 +143,18 @@ int par_svbp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbat
     P.cents = cent_s(g);
     P.top = static_cast<double>(g.ns) - 1;
     const dim3 grid(static_cast<unsigned int>((npix + kSvPix - 1) / kSvPix), nbatch);
+    const size_t esize = dtype == CTR_F64 ? 8 : 4;
+    const auto a16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    const int vec = (g.na * 8) % 16 == 0 && (g.na * esize) % 16 == 0 && a16(img) && a16(xcos) &&
+                    a16(ysin);
     if (dtype == CTR_F64)
         hipLaunchKernelGGL(par_svbp_kernel<double>, grid, dim3(256), 0, st,
                            static_cast<const double*>(proj), static_cast<double*>(img), xcos,
-                           ysin, P, static_cast<int>(g.nx));
+                           ysin, P, static_cast<int>(g.nx), vec);
     else
         hipLaunchKernelGGL(par_svbp_kernel<float>, grid, dim3(256), 0, st,
                            static_cast<const float*>(proj), static_cast<float*>(img), xcos, ysin,
-                           P, static_cast<int>(g.nx));
+                           P, static_cast<int>(g.nx), vec);
     return check_launch("par_svbp_kernel");
 }
 

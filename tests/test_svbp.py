@@ -46,17 +46,21 @@ GEO = dict(ns=90, na=100, ds=1.1, orbit=math.pi / 100, offset_s=0.35, orbit_star
            nx=97, ny=61, dx=1.3, dy=1.7, offset_x=-0.4, offset_y=0.6)
 
 
+# na = 37: a view count whose rows are not 16-byte multiples (element-wise
+# table reads and stores instead of 16-byte ones).
+@pytest.mark.parametrize("na", [100, 37])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_svbp_batch_bit_exact_vs_oracle(F, dtype):
-    g = O.geom(**GEO)
+def test_svbp_batch_bit_exact_vs_oracle(F, dtype, na):
+    geo = dict(GEO, na=na, orbit=math.pi / na)
+    g = O.geom(**geo)
     t = O.par_prep(g)
     proj = randn(77, (3, g.na, g.ns), dtype)
-    out = F.parallel_single_view_bp_2d(torch.from_numpy(proj).cuda()[..., None], **kwargs(GEO))
+    out = F.parallel_single_view_bp_2d(torch.from_numpy(proj).cuda()[..., None], **kwargs(geo))
     assert out.shape == (3, g.ny, g.nx, g.na)
     out = out.cpu().numpy()
     for b in range(3):
         ref = O.par_svbp(g, proj[b], t)
-        assert (ref == 0).sum() > 1000 and (ref != 0).sum() > 1000
+        assert (ref == 0).sum() > 300 and (ref != 0).sum() > 1000
         np.testing.assert_array_equal(out[b], ref, err_msg=f"slice {b}")
 
 
