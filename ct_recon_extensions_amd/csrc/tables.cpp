@@ -22,6 +22,14 @@ using ctr::cent_s;
 using ctr::cent_x;
 using ctr::cent_y;
 
+// sin and cos of one angle through glibc's sincos.  GCC at -O2/-O3 (the
+// reference's build, CMakeLists.txt:22) turns a sin and a cos of the same
+// argument into one sincos call, whose cosine can differ from cos() in the
+// last bit (view 464 of 720 over 180 deg is one such angle); calling it
+// explicitly keeps these tables equal to the reference's whichever compiler
+// builds this file.
+static inline void sin_cos(double x, double& s, double& c) { ::sincos(x, &s, &c); }
+
 extern "C" {
 
 const char* ctr_last_error(void) { return ctr::get_error(); }
@@ -48,8 +56,7 @@ int ctr_fan_tables(const ctr_geom* g, double* xpos, double* ypos, double* sincos
     CTR_REQUIRE(g && xpos && ypos && sincostbl, CTR_ERR_ARG, "ctr_fan_tables: null argument");
     double angle = g->orbit_start;
     for (unsigned int a = 0; a < g->na; ++a) {
-        sincostbl[2 * a] = std::sin(angle);
-        sincostbl[2 * a + 1] = std::cos(angle);
+        sin_cos(angle, sincostbl[2 * a], sincostbl[2 * a + 1]);
         angle += g->orbit;
     }
     pixel_positions(*g, xpos, ypos);
@@ -62,8 +69,7 @@ int ctr_fan_angle_tables(const ctr_geom* g, const double* angles, double* xpos, 
     CTR_REQUIRE(g && angles && xpos && ypos && sincostbl, CTR_ERR_ARG,
                 "ctr_fan_angle_tables: null argument");
     for (unsigned int a = 0; a < g->na; ++a) {
-        sincostbl[2 * a] = std::sin(angles[a]);
-        sincostbl[2 * a + 1] = std::cos(angles[a]);
+        sin_cos(angles[a], sincostbl[2 * a], sincostbl[2 * a + 1]);
     }
     pixel_positions(*g, xpos, ypos);
     return CTR_OK;
@@ -76,7 +82,8 @@ int ctr_par_tables(const ctr_geom* g, double* xcos, double* ysin)
     CTR_REQUIRE(g && xcos && ysin, CTR_ERR_ARG, "ctr_par_tables: null argument");
     double angle = g->orbit_start;
     for (unsigned int a = 0; a < g->na; ++a) {
-        const double sa = std::sin(angle), ca = std::cos(angle);
+        double sa, ca;
+        sin_cos(angle, sa, ca);
         double px = -cent_x(*g) * g->dx;
         for (unsigned int i = 0; i < g->nx; ++i) {
             xcos[a + static_cast<size_t>(i) * g->na] = px * ca;
@@ -106,7 +113,8 @@ int ctr_fp_tables(const ctr_geom* g, double* sincostbl, double* beginoffset, int
     for (unsigned int a = 0; a < g->na; ++a) {
         while (angle < 0) angle += 2 * M_PI;
         while (angle >= 2 * M_PI) angle -= 2 * M_PI;
-        const double sa = std::sin(angle), ca = std::cos(angle);
+        double sa, ca;
+        sin_cos(angle, sa, ca);
         sincostbl[2 * a] = sa;
         sincostbl[2 * a + 1] = ca;
         const bool ybranch = (angle >= q && angle < 3 * q) || (angle >= 5 * q && angle < 7 * q);
@@ -145,7 +153,8 @@ int ctr_fp_grad_tables(const ctr_geom* g, double* weights, double* pos, int* use
     const double q = M_PI / 4;
     double angle = g->orbit_start;
     for (unsigned int a = 0; a < g->na; ++a) {
-        const double sa = std::sin(angle), ca = std::cos(angle);
+        double sa, ca;
+        sin_cos(angle, sa, ca);
         const bool ybranch = (angle >= q && angle < 3 * q) || (angle >= 5 * q && angle < 7 * q);
         usex[a] = ybranch ? 0 : 1;
         double* p = pos + 2 * static_cast<size_t>(a) * pos_stride;

@@ -30,8 +30,12 @@
  *
  * Build: oracle/Makefile -> oracle/libct_oracle.so (gcc -O2, C99, no FMA
  * contraction, so the FP64 expressions round exactly like the reference's
- * g++ -O3 -std=c++11 build on x86-64).
+ * g++ -O3 -std=c++11 build on x86-64).  Sines and cosines of one angle go
+ * through glibc's sincos: g++ -O3 merges the reference's sin(angle) and
+ * cos(angle) into that call, and its cosine can differ from cos() in the
+ * last bit (view 464 of 720 over 180 deg).
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stddef.h>
 #include <string.h>
@@ -67,8 +71,7 @@ int orc_fan_prep(const orc_geom* g, double* xpos, double* ypos, double* sincostb
     double posx = -centx * g->dx, posy = centy * g->dy;
     unsigned int i;
     for (i = 0; i < g->na; ++i) {
-        sincostbl[2 * i] = sin(angle);
-        sincostbl[2 * i + 1] = cos(angle);
+        sincos(angle, &sincostbl[2 * i], &sincostbl[2 * i + 1]);
         angle += g->orbit;
     }
     for (i = 0; i < g->nx; ++i) { xpos[i] = posx; posx += g->dx; }
@@ -85,8 +88,7 @@ int orc_fan_angle_prep(const orc_geom* g, const double* angles, double* xpos,
     double posx = -centx * g->dx, posy = centy * g->dy;
     unsigned int i;
     for (i = 0; i < g->na; ++i) {
-        sincostbl[2 * i] = sin(angles[i]);
-        sincostbl[2 * i + 1] = cos(angles[i]);
+        sincos(angles[i], &sincostbl[2 * i], &sincostbl[2 * i + 1]);
     }
     for (i = 0; i < g->nx; ++i) { xpos[i] = posx; posx += g->dx; }
     for (i = 0; i < g->ny; ++i) { ypos[i] = posy; posy -= g->dy; }
@@ -192,7 +194,8 @@ int orc_par_prep(const orc_geom* g, double* xcos, double* ysin)
     double angle = g->orbit_start;
     unsigned int ia, ix, iy;
     for (ia = 0; ia < g->na; ++ia) {
-        const double sa = sin(angle), ca = cos(angle);
+        double sa, ca;
+        sincos(angle, &sa, &ca);
         double posx = -centx * g->dx, posy = centy * g->dy;
         for (ix = 0; ix < g->nx; ++ix) { xcos[ia + ix * g->na] = posx * ca; posx += g->dx; }
         for (iy = 0; iy < g->ny; ++iy) { ysin[ia + iy * g->na] = posy * sa; posy -= g->dy; }
@@ -318,8 +321,7 @@ int orc_fp_prep(const orc_geom* g, double* sincostbl, double* beginoffset, int* 
         double sa, ca, offset, offset2, begin;
         while (angle < 0) angle += 2 * M_PI;
         while (angle >= 2 * M_PI) angle -= 2 * M_PI;
-        sa = sin(angle);
-        ca = cos(angle);
+        sincos(angle, &sa, &ca);
         sincostbl[2 * ia] = sa;
         sincostbl[2 * ia + 1] = ca;
         if ((angle >= M_PI_4 && angle < 3 * M_PI_4) ||
@@ -416,7 +418,8 @@ int orc_fp_grad_prep(const orc_geom* g, double* weights, double* pos, int* usex)
     unsigned int ia, k;
     double* pp = pos;
     for (ia = 0; ia < g->na; ++ia) {
-        const double sa = sin(angle), ca = cos(angle);
+        double sa, ca;
+        sincos(angle, &sa, &ca);
         const int bx = !((angle >= M_PI_4 && angle < 3 * M_PI_4) ||
                          (angle >= 5 * M_PI_4 && angle < 7 * M_PI_4));
         double o1, o2, begin;

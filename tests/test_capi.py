@@ -142,3 +142,28 @@ def test_python_ops_refuse_cpu_tensors(lib):
     import ct_recon_extensions_amd as F
     with pytest.raises(ValueError, match="GPU tensor"):
         F.fan_backprojection_2d(torch.zeros(1, 4, 8), [4, 4], [1, 1], [4, 8], 1.0)
+
+
+def test_tables_equal_oracle_at_full_size():
+    """The host tables at the workloads' sizes equal the oracle's (built by gcc
+    like the reference, whose -O2/-O3 turns sin and cos of one angle into
+    sincos; view 464 of 720 over 180 deg is an angle where sincos and cos
+    give cosines one ulp apart)."""
+    import ct_recon_extensions_amd.geometry as G
+    n, na, ns = 512, 720, 729
+    q = dict(ns=ns, na=na, ds=1.0, orbit=math.pi / na, nx=n, ny=n, dx=1.0, dy=1.0)
+    g = O.geom(**q)
+    pp = G.ParallelBackprojection2DParam(ns, na, 1.0, math.pi / na, 0.0, 0.0, n, n, 1.0, 1.0,
+                                         0.0, 0.0)
+    for a, b in zip(G.par_tables(pp), O.par_prep(g)):
+        np.testing.assert_array_equal(a, b)
+    pf = G.ParallelProjection2DParam(ns, na, 1.0, math.pi / na, 0.0, 0.0, n, n, 1.0, 1.0,
+                                     0.0, 0.0)
+    for a, b in zip(G.fp_tables(pf), O.fp_prep(g)):
+        np.testing.assert_array_equal(a, b)
+    gf = O.geom(ns=736, na=720, ds=1.30, orbit=2 * math.pi / 720, nx=512, ny=512, dx=0.7,
+                dy=0.7, dso=595.0, dsd=1085.6)
+    pfan = G.FanBackprojection2DParam(736, 720, 1.30, 2 * math.pi / 720, 0.0, 0.0, 512, 512,
+                                      0.7, 0.7, 0.0, 0.0, 595.0, 1085.6)
+    for a, b in zip(G.fan_tables(pfan), O.fan_prep(gf)):
+        np.testing.assert_array_equal(a, b)
