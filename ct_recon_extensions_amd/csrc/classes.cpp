@@ -16,6 +16,7 @@
 #include "include/bp_fan_2d.h"
 #include "include/bp_fan_2d_angle.h"
 #include "include/bp_par_2d.h"
+#include "include/bp_par_2d_sv.h"
 #include "include/fan_weighting.h"
 #include "include/filter.h"
 #include "include/fp_par_2d.h"
@@ -318,7 +319,10 @@ bool ParallelBackprojection2DPixDrivenGradPrep::calculate_on_cpu(double* begins,
     const double q = M_PI / 4;
     double angle = g.orbit_start;
     for (unsigned int a = 0; a < g.na; ++a) {
-        const double sa = std::sin(angle), ca = std::cos(angle);
+        // one sincos call, as the reference's g++ -O3 build makes of its sin
+        // and cos of the angle (the cosine can differ from cos() by an ulp)
+        double sa, ca;
+        ::sincos(angle, &sa, &ca);
         while (angle < 0) angle += 2 * M_PI;
         while (angle >= 2 * M_PI) angle -= 2 * M_PI;
         const bool bx = !((angle >= q && angle < 3 * q) || (angle >= 5 * q && angle < 7 * q));
@@ -354,6 +358,42 @@ bool ParallelBackprojection2DPixDrivenGradPrep::calculate_on_gpu(double* begins,
     return to_device(begins, b, st) && to_device(offsets, o, st) && to_device(usex, u, st) &&
            ok_sync(CTR_OK, st);
 }
+
+bool ParallelSingleViewBp2DPixDrivenPrep::calculate_on_cpu(double* xcos, double* ysin,
+                                                           int*) const
+{
+    const ctr_geom g = geom_par(param_);
+    return ctr_par_tables(&g, xcos, ysin) == CTR_OK;
+}
+
+bool ParallelSingleViewBp2DPixDrivenPrep::calculate_on_gpu(double* xcos, double* ysin, int*,
+                                                           hipStream_t st) const
+{
+    const ctr_geom g = geom_par(param_);
+    std::vector<double> xc(static_cast<size_t>(g.na) * g.nx), ys(static_cast<size_t>(g.na) * g.ny);
+    if (ctr_par_tables(&g, xc.data(), ys.data()) != CTR_OK) return false;
+    return to_device(xcos, xc, st) && to_device(ysin, ys, st) && ok_sync(CTR_OK, st);
+}
+
+template <typename T>
+bool ParallelSingleViewBp2DPixDriven<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                          const double*, const int*) const
+{
+    return no_cpu("ParallelSingleViewBp2DPixDriven::calculate_on_cpu");
+}
+
+template <typename T>
+bool ParallelSingleViewBp2DPixDriven<T>::calculate_on_gpu(const T* proj, T* img,
+                                                          const double* xcos,
+                                                          const double* ysin, const int*,
+                                                          hipStream_t st) const
+{
+    const ctr_geom g = geom_par(this->param_);
+    return ok_sync(ctr_par_sv_bp(&g, dtype_of<T>(), proj, img, 1, xcos, ysin, st), st);
+}
+
+template class ParallelSingleViewBp2DPixDriven<float>;
+template class ParallelSingleViewBp2DPixDriven<double>;
 
 template <typename T>
 bool ParallelBackprojection2DPixDriven<T>::calculate_on_cpu(const T*, T*, const double*,

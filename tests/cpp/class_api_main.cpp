@@ -16,6 +16,7 @@
 #include "include/bp_fan_2d.h"
 #include "include/bp_fan_2d_angle.h"
 #include "include/bp_par_2d.h"
+#include "include/bp_par_2d_sv.h"
 #include "include/fan_weighting.h"
 #include "include/filter.h"
 #include "include/fp_par_2d.h"
@@ -79,6 +80,8 @@ static const FanBackprojection2DAngleParam kFanA(41, 48, 1.7, 0.4, 24, 20, 1.3, 
                                                  60.0, 110.0, -1.0);
 static const ParallelBackprojection2DParam kPar(41, 36, 0.8, M_PI / 36, -0.35, 0.1, 22, 26, 0.7,
                                                 0.9, -0.4, 0.25, -1.0);
+static const ParallelSingleViewBp2DParam kSv(41, 36, 0.8, M_PI / 36, -0.35, 0.1, 22, 26, 0.7,
+                                              0.9, -0.4, 0.25, -1.0);
 static const ParallelProjection2DParam kFp(41, 40, 1.0, M_PI / 40, 0.3, 0.0, 24, 24, 1.0, 1.0,
                                            0.2, -0.1, -1.0);
 static const FilterParam kFilt(33, 5, 0.9, 40.0, 1, 0);
@@ -92,6 +95,9 @@ static int cpu()
     std::vector<double> xc(kPar.na * kPar.nx), ys(kPar.na * kPar.ny);
     CHECK(ParallelBackprojection2DPixDrivenPrep(kPar).calculate_on_cpu(xc.data(), ys.data(), nullptr));
     save("par_xcos", xc); save("par_ysin", ys);
+    std::vector<double> sxc(kSv.na * kSv.nx), sys(kSv.na * kSv.ny);
+    CHECK(ParallelSingleViewBp2DPixDrivenPrep(kSv).calculate_on_cpu(sxc.data(), sys.data(), nullptr));
+    save("svbp_xcos", sxc); save("svbp_ysin", sys);
     std::vector<double> s2(2 * kFp.na), bo(2 * kFp.na * kFp.ns);
     std::vector<int> u(kFp.na);
     CHECK(ParallelProjection2DRayDrivenPrepare(kFp).calculate_on_cpu(s2.data(), bo.data(), u.data()));
@@ -145,6 +151,13 @@ static int gpu_typed(const char* suf, hipStream_t st)
         Dev<T> im2(load("par_img", kPar.nx * kPar.ny)), grad(kPar.na * kPar.ns);
         CHECK(ParallelBackprojection2DPixDrivenGrad<T>(kPar).calculate_on_gpu(im2.p, grad.p, b.p, o.p, u.p, st));
         save("par_bpt_" + s, grad.get());
+    }
+    {   // single-view parallel BP (the parallel geometry)
+        Dev<double> xc(kSv.na * kSv.nx), ys(kSv.na * kSv.ny);
+        CHECK(ParallelSingleViewBp2DPixDrivenPrep(kSv).calculate_on_gpu(xc.p, ys.p, nullptr, st));
+        Dev<T> proj(load("par_proj", kSv.na * kSv.ns)), img(kSv.nx * kSv.ny * kSv.na);
+        CHECK(ParallelSingleViewBp2DPixDriven<T>(kSv).calculate_on_gpu(proj.p, img.p, xc.p, ys.p, nullptr, st));
+        save("svbp_" + s, img.get());
     }
     {   // ray-driven FP and its transpose
         Dev<double> sc(2 * kFp.na), bo(2 * kFp.na * kFp.ns);

@@ -46,6 +46,8 @@ def test_class_api_prep_tables_cpu(driver, tmp_path):
         np.testing.assert_array_equal(_read(tmp_path, name), a)
     for a, name in zip(O.par_prep(O.geom(**PAR)), ("par_xcos", "par_ysin")):
         np.testing.assert_array_equal(_read(tmp_path, name), a)
+    for a, name in zip(O.par_svbp_prep(O.geom(**PAR)), ("svbp_xcos", "svbp_ysin")):
+        np.testing.assert_array_equal(_read(tmp_path, name), a)
     for a, name in zip(O.fp_prep(O.geom(**FP)), ("fp_sincos", "fp_beginoffset", "fp_usex")):
         np.testing.assert_array_equal(_read(tmp_path, name), a)
     for a, name in zip(O.fp_grad_prep(O.geom(**FP)), ("fpg_weights", "fpg_pos", "fpg_usex")):
@@ -91,6 +93,10 @@ def test_class_api_operators_gpu(driver, tmp_path):
             "fw": O.fanw(gw, inputs["fw_in"].astype(dt)),
             "fwg": O.fanw(gw, inputs["fw_in"].astype(dt), grad=True),
         }
+        # single-view BP: bit-identical to the reference's arithmetic
+        np.testing.assert_array_equal(
+            _read(tmp_path, f"svbp_{suf}"),
+            O.par_svbp(gp, inputs["par_proj"].astype(dt), tp).ravel().astype(np.float64))
         for name, want in ref.items():
             got = _read(tmp_path, f"{name}_{suf}")
             # the float transposes scatter with atomics; the FP64 ones too
