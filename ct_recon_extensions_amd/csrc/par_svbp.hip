@@ -85,21 +85,34 @@ __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ pro
         }
         __syncthreads();
         if (c_ok) {
+            // positions and taps of all 8 views first, then the 16 gathers
+            // back to back (an unused view reads channel 0 of a valid row)
+            double u[8];
+            int i1[8], i2[8];
+            bool ok[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const int v = cv + j;
-                if (v >= nv) break;
+                const int v = min(cv + j, nv - 1);
                 const double s = pos[cp][v] / P.ds + P.cents;
-                T r = T(0);
-                if (s >= 0 && s <= P.top) {
-                    const double f = floor(s);
-                    const int is1 = static_cast<int>(f), is2 = static_cast<int>(ceil(s));
-                    const double u = s - f;
-                    const T* row = pr + static_cast<size_t>(va + v) * P.ns;
-                    r = static_cast<T>((1 - u) * static_cast<double>(row[is1]) +
-                                       u * static_cast<double>(row[is2]));
-                }
-                res[cp][v] = r;
+                ok[j] = cv + j < nv && s >= 0 && s <= P.top;
+                const double f = floor(s);
+                i1[j] = ok[j] ? static_cast<int>(f) : 0;
+                i2[j] = ok[j] ? static_cast<int>(ceil(s)) : 0;
+                u[j] = s - f;
+            }
+            T p0[8], p1[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const T* row = pr + static_cast<size_t>(va + min(cv + j, nv - 1)) * P.ns;
+                p0[j] = row[i1[j]];
+                p1[j] = row[i2[j]];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const T r = ok[j] ? static_cast<T>((1 - u[j]) * static_cast<double>(p0[j]) +
+                                                   u[j] * static_cast<double>(p1[j]))
+                                  : T(0);
+                if (cv + j < nv) res[cp][cv + j] = r;
             }
         }
         __syncthreads();
