@@ -58,6 +58,7 @@ _SIGS = {
     "ctr_fan_angle_tables": (C.c_int, [_G, _P, _P, _P, _P]),
     "ctr_par_tables": (C.c_int, [_G, _P, _P]),
     "ctr_fp_tables": (C.c_int, [_G, _P, _P, _P]),
+    "ctr_fp_rc_tables": (C.c_int, [_G, C.c_double, _P, _P, _P]),
     "ctr_fp_grad_tables": (C.c_int, [_G, _P, _P, _P, C.c_uint]),
     "ctr_ramp_taps": (C.c_int, [_G, _P]),
     "ctr_fan_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, _P]),
@@ -68,6 +69,7 @@ _SIGS = {
     "ctr_par_bp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P]),
     "ctr_par_sv_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
     "ctr_fp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
+    "ctr_fp_rc": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, C.c_double, _P, _P, _P, _P]),
     "ctr_fp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_uint, _P]),
     "ctr_ramp": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, _P]),
     "ctr_fan_weighting": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P]),

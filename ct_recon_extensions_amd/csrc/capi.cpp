@@ -25,6 +25,8 @@ int par_bpt(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatc
             const double* xcos, const double* ysin, bool det, hipStream_t st);
 int par_svbp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
              const double* xcos, const double* ysin, hipStream_t st);
+int fp_rc(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch, double step,
+          const double* sincostbl, const double* begins, const int* nsteps, hipStream_t st);
 int fp_par(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
            const double* sc, const double* bo, const int* usex, hipStream_t st);
 int fp_par_t(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
@@ -178,6 +180,19 @@ int ctr_fp(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch
     return guarded("ctr_fp", [&] {
         return ctr::fp_par(*g, dtype, img, proj, nbatch, sincostbl, beginoffset, usex,
                            static_cast<hipStream_t>(stream));
+    });
+}
+
+int ctr_fp_rc(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
+              double step, const double* sincostbl, const double* begins, const int* nsteps,
+              void* stream)
+{
+    int rc = check_common(g, dtype, img, proj, nbatch, "ctr_fp_rc");
+    if (rc) return rc;
+    CTR_REQUIRE(sincostbl && begins && nsteps, CTR_ERR_ARG, "ctr_fp_rc: null table");
+    return guarded("ctr_fp_rc", [&] {
+        return ctr::fp_rc(*g, dtype, img, proj, nbatch, step, sincostbl, begins, nsteps,
+                          static_cast<hipStream_t>(stream));
     });
 }
 

@@ -99,6 +99,54 @@ int ctr_par_tables(const ctr_geom* g, double* xcos, double* ysin)
     return CTR_OK;
 }
 
+// Ray-casting FP tables (src/fp_par_2d.cc:18-89): per view (sin, cos) of
+// the accumulated angle; per ray the entry point on the FOV circle (radius
+// fov) in pixel-index units and the number of steps of length `step` along
+// the marched axis (x when sin > 1/1.414); rays missing the circle: 0 steps.
+int ctr_fp_rc_tables(const ctr_geom* g, double step, double* sincostbl, double* begins,
+                     int* nsteps)
+{
+    CTR_REQUIRE(g && sincostbl && begins && nsteps, CTR_ERR_ARG,
+                "ctr_fp_rc_tables: null argument");
+    CTR_REQUIRE(step > 0, CTR_ERR_ARG, "ctr_fp_rc_tables: step must be positive");
+    const double xc = static_cast<double>(g->nx - 1) / 2.0 + g->offset_x;
+    const double yc = static_cast<double>(g->ny - 1) / 2.0 + g->offset_y;
+    const double fov2 = g->fov * g->fov;
+    const double s0 = -(static_cast<double>(g->ns - 1) / 2.0 + g->offset_s) * g->ds;
+    double angle = g->orbit_start;
+    for (unsigned int a = 0; a < g->na; ++a) {
+        double sa, ca;
+        sin_cos(angle, sa, ca);
+        sincostbl[2 * a] = sa;
+        sincostbl[2 * a + 1] = ca;
+        const double stx = step * sa, sty = step * ca;
+        const bool usex = sa > (1.0 / 1.414);
+        double sv = s0;
+        for (unsigned int i = 0; i < g->ns; ++i) {
+            const size_t r = static_cast<size_t>(a) * g->ns + i;
+            const double h2 = fov2 - sv * sv;
+            if (h2 > 0) {
+                const double h = std::sqrt(h2);
+                const double x1 = (sv * ca - h * sa) / g->dx + xc;
+                const double y1 = -(sv * sa + h * ca) / g->dy + yc;
+                const double x2 = (sv * ca + h * sa) / g->dx + xc;
+                const double y2 = -(sv * sa - h * ca) / g->dy + yc;
+                nsteps[r] = usex ? static_cast<int>((x2 - x1) / stx)
+                                 : static_cast<int>((y2 - y1) / sty);
+                begins[2 * r] = x1;
+                begins[2 * r + 1] = y1;
+            } else {
+                nsteps[r] = 0;
+                begins[2 * r] = 0;
+                begins[2 * r + 1] = 0;
+            }
+            sv += g->ds;
+        }
+        angle += g->orbit;
+    }
+    return CTR_OK;
+}
+
 // Ray-driven FP tables (src/fp_par_2d.cc:149-208): the angle is wrapped into
 // [0, 2pi) before use, the x-branch is used outside [pi/4,3pi/4) and
 // [5pi/4,7pi/4), and each channel's start position advances by repeated

@@ -23,9 +23,9 @@ from typing import Sequence
 import torch
 
 from . import _lib
-from .geometry import (FanBackprojection2DAngleParam, FanBackprojection2DParam, FanWeightingParam,
-                       FilterParam, ParallelBackprojection2DParam, ParallelProjection2DParam,
-                       device_tables)
+from .geometry import (RC_STEP, FanBackprojection2DAngleParam, FanBackprojection2DParam,
+                       FanWeightingParam, FilterParam, ParallelBackprojection2DParam,
+                       ParallelProjection2DParam, device_tables)
 
 __all__ = [
     "ramp_filter", "ramp_filter_grad", "parallel_backprojection_2d",
@@ -335,13 +335,23 @@ class _FPT(torch.autograd.Function):
 def parallel_projection_2d(img, img_shape, img_space, proj_shape, channel_space,
                            img_offset=(0, 0), channel_offset=0, orbit_start=0,
                            orbit=math.pi / 180, fov=-1, method="raydriven"):
-    """Ray-driven parallel FP (python/tf_ctrecon.py:173-195;
-    src/fp_par_2d.cc:210-280).  img [N, ny, nx, 1] -> proj [N, na, ns, 1].
-    Backward: the gather-form transpose (src/fp_par_2d.cc:332-391)."""
-    _check_method(method, ("raydriven",), "parallel_projection_2d")
+    """Parallel FP (python/tf_ctrecon.py:173-195).  img [N, ny, nx, 1] ->
+    proj [N, na, ns, 1].  method "raydriven" (src/fp_par_2d.cc:210-280;
+    backward: the gather-form transpose, src/fp_par_2d.cc:332-391) or
+    "raycasting" (src/fp_par_2d.cc:18-144: rays from their entry points on
+    the FOV circle of radius fov, step 0.125, bilinear samples; bit-identical
+    to the reference; like the reference it has no gradient)."""
+    _check_method(method, ("raydriven", "raycasting"), "parallel_projection_2d")
     p = _par_param(img_shape, img_space, proj_shape, channel_space, img_offset, channel_offset,
                    orbit_start, orbit, fov, cls=ParallelProjection2DParam)
     x, restore = _to_batch(img, p.ny, p.nx, "img")
+    if method == "raycasting":
+        sc, begins, nsteps = device_tables("fp_rc", p, x.device)
+        n = x.shape[0]
+        out = torch.empty((n, p.na, p.ns), dtype=x.dtype, device=x.device)
+        _call_batched("ctr_fp_rc", n, C.byref(p.ctr()), _dtype_code(x), _ptr(x), _ptr(out), n,
+                      RC_STEP, _ptr(sc), _ptr(begins), _ptr(nsteps), _stream(x))
+        return restore(out)
     return restore(_FP.apply(x, p, device_tables("fp", p, x.device),
                              device_tables("fp_grad", p, x.device)))
 
@@ -350,6 +360,8 @@ def parallel_projection_2d_grad(grad_in, img_shape, img_space, proj_shape, chann
                                 img_offset=(0, 0), channel_offset=0, orbit_start=0,
                                 orbit=math.pi / 180, fov=-1, method="raydriven"):
     """Transpose of parallel_projection_2d (tensorflow/fp_par_2d_ops.cu:245-374)."""
+    if method == "raycasting":  # tensorflow/fp_par_2d_ops.cu:287-291
+        raise RuntimeError("Gradient is not yet implemented for raycasting forward projector.")
     _check_method(method, ("raydriven",), "parallel_projection_2d_grad")
     p = _par_param(img_shape, img_space, proj_shape, channel_space, img_offset, channel_offset,
                    orbit_start, orbit, fov, cls=ParallelProjection2DParam)

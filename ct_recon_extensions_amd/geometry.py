@@ -165,6 +165,17 @@ def fp_tables(p: ParallelProjection2DParam):
     return sc, bo, usex
 
 
+RC_STEP = 0.125  # the reference's ray-casting step (include/fp_par_2d.h:160,186)
+
+
+def fp_rc_tables(p: ParallelProjection2DParam, step: float = RC_STEP):
+    sc, begins = np.zeros(2 * p.na), np.zeros(2 * p.na * p.ns)
+    nsteps = np.zeros(p.na * p.ns, np.int32)
+    _lib.call("ctr_fp_rc_tables", C.byref(p.ctr()), float(step), _ptr(sc), _ptr(begins),
+              _ptr(nsteps))
+    return sc, begins, nsteps
+
+
 def fp_grad_tables(p: ParallelProjection2DParam, stride: int | None = None):
     stride = max(p.nx, p.ny) if stride is None else stride
     w, pos = np.zeros(p.na), np.zeros(2 * p.na * stride)
@@ -204,6 +215,8 @@ def device_tables(kind: str, p, device, angles=None):
         host = par_tables(p)
     elif kind == "fp":
         host = fp_tables(p)
+    elif kind == "fp_rc":
+        host = fp_rc_tables(p)
     elif kind == "fp_grad":
         host = fp_grad_tables(p)
     elif kind == "ramp":

@@ -81,6 +81,12 @@ int ctr_par_tables(const ctr_geom* g, double* xcos, double* ysin);
 /* ParallelProjection2DRayDrivenPrepare (src/fp_par_2d.cc:149-208):
  * sincostbl[2na], beginoffset[na][ns][2], usex[na]. */
 int ctr_fp_tables(const ctr_geom* g, double* sincostbl, double* beginoffset, int* usex);
+/* ParallelProjection2DRayCastingPrepare (src/fp_par_2d.cc:18-89), the
+ * reference's default step 0.125: sincostbl[na][2], begins[na][ns][2] (entry
+ * point on the FOV circle of radius fov, pixel-index units), nsteps[na][ns]
+ * (0 for rays missing the circle). */
+int ctr_fp_rc_tables(const ctr_geom* g, double step, double* sincostbl, double* begins,
+                     int* nsteps);
 /* ParallelProjection2DRayDrivenGradPrep (src/fp_par_2d.cc:285-330):
  * weights[na], pos[na][pos_stride][2], usex[na].  The reference uses
  * pos_stride == nx (and so requires nx == ny); pos_stride >= max(nx, ny)
@@ -141,6 +147,14 @@ int ctr_par_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int 
  * (src/fp_par_2d.cc:210-280): img [nbatch][ny][nx] -> proj [nbatch][na][ns]. */
 int ctr_fp(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
            const double* sincostbl, const double* beginoffset, const int* usex, void* stream);
+/* Ray-casting parallel FP, ParallelProjection2DRayCasting<T>
+ * (src/fp_par_2d.cc:91-144; TF FP op with method "raycasting",
+ * tensorflow/fp_par_2d_ops.cu:152-161): img [nbatch][ny][nx] -> proj
+ * [nbatch][na][ns] from the tables of ctr_fp_rc_tables; bit-identical to
+ * the reference's CPU path.  The reference has no transpose for it. */
+int ctr_fp_rc(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
+              double step, const double* sincostbl, const double* begins, const int* nsteps,
+              void* stream);
 /* Gather-form transpose of ctr_fp, ParallelProjection2DRayDrivenGrad<T>
  * (src/fp_par_2d.cc:332-391), channel range clamped to the detector. */
 int ctr_fp_t(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,

@@ -234,6 +234,97 @@ int orc_par_bp_##SUF(const orc_geom* g, const T* proj, T* img, const double* xco
 DEF_PAR_BP(float, f)
 DEF_PAR_BP(double, d)
 
+/* Ray-casting parallel FP tables, ParallelProjection2DRayCastingPrepare
+ * (src/fp_par_2d.cc:18-89): per view (sin, cos) of the accumulated angle;
+ * per ray the entry point (x1, y1) in pixel-index units on the FOV circle of
+ * radius fov and the number of steps of length `step` (index units along the
+ * marched axis: x when sin > 1/1.414, else y); rays missing the circle get 0
+ * steps and a zero entry point. */
+int orc_fp_rc_prep(const orc_geom* g, double step, double* sincostbl, double* begins,
+                   int* nsteps)
+{
+    const double xc = (double)(g->nx - 1) / 2.0 + g->offset_x;
+    const double yc = (double)(g->ny - 1) / 2.0 + g->offset_y;
+    const double fov2 = g->fov * g->fov;
+    const double s0 = -((double)(g->ns - 1) / 2.0 + g->offset_s) * g->ds;
+    double angle = g->orbit_start;
+    unsigned int ia, is;
+    for (ia = 0; ia < g->na; ++ia) {
+        double sa, ca, s = s0;
+        sincos(angle, &sa, &ca);
+        sincostbl[2 * ia] = sa;
+        sincostbl[2 * ia + 1] = ca;
+        {
+            const double stx = step * sa, sty = step * ca;
+            const int usex = sa > (1.0 / 1.414);
+            for (is = 0; is < g->ns; ++is) {
+                const size_t r = (size_t)ia * g->ns + is;
+                const double h2 = fov2 - s * s;
+                if (h2 > 0) {
+                    const double h = sqrt(h2);
+                    const double x1 = (s * ca - h * sa) / g->dx + xc;
+                    const double y1 = -(s * sa + h * ca) / g->dy + yc;
+                    const double x2 = (s * ca + h * sa) / g->dx + xc;
+                    const double y2 = -(s * sa - h * ca) / g->dy + yc;
+                    nsteps[r] = usex ? (int)((x2 - x1) / stx) : (int)((y2 - y1) / sty);
+                    begins[2 * r] = x1;
+                    begins[2 * r + 1] = y1;
+                } else {
+                    nsteps[r] = 0;
+                    begins[2 * r] = 0;
+                    begins[2 * r + 1] = 0;
+                }
+                s += g->ds;
+            }
+        }
+        angle += g->orbit;
+    }
+    return 1;
+}
+
+/* Ray-casting parallel FP, ParallelProjection2DRayCasting<T>
+ * (src/fp_par_2d.cc:91-144): from the entry point, nsteps steps of
+ * (step sin, step cos) in index units; samples inside [0, nx-1] x [0, ny-1]
+ * add the bilinear interpolation of the image (FP64 sum), and the ray is
+ * sum * step * dx. */
+#define DEF_FP_RC(T, SUF)                                                         \
+int orc_fp_rc_##SUF(const orc_geom* g, double step, const T* img, T* proj,       \
+                    const double* sincostbl, const double* begins, const int* nsteps) \
+{                                                                                 \
+    unsigned int ia, is;                                                          \
+    for (ia = 0; ia < g->na; ++ia) {                                              \
+        const double sa = sincostbl[2 * ia], ca = sincostbl[2 * ia + 1];          \
+        const double stx = step * sa, sty = step * ca;                            \
+        for (is = 0; is < g->ns; ++is) {                                          \
+            const size_t r = (size_t)ia * g->ns + is;                             \
+            const int n = nsteps[r];                                              \
+            if (n > 0) {                                                          \
+                double x = begins[2 * r], y = begins[2 * r + 1], sum = 0;         \
+                int k;                                                            \
+                for (k = 0; k < n; ++k) {                                         \
+                    if (x >= 0 && x <= g->nx - 1 && y >= 0 && y <= g->ny - 1) {   \
+                        const int ix1 = (int)floor(x), ix2 = (int)ceil(x);        \
+                        const int iy1 = (int)floor(y), iy2 = (int)ceil(y);        \
+                        const double u = x - ix1, v = y - iy1;                    \
+                        sum += (1 - v) * ((1 - u) * img[ix1 + iy1 * g->nx]        \
+                                          + u * img[ix2 + iy1 * g->nx])           \
+                               + v * ((1 - u) * img[ix1 + iy2 * g->nx]            \
+                                      + u * img[ix2 + iy2 * g->nx]);              \
+                    }                                                             \
+                    x += stx;                                                     \
+                    y += sty;                                                     \
+                }                                                                 \
+                proj[r] = (T)(sum * step * g->dx);                                \
+            } else {                                                              \
+                proj[r] = 0;                                                      \
+            }                                                                     \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_FP_RC(float, f)
+DEF_FP_RC(double, d)
+
 /* Single-view parallel BP, src/bp_par_2d_sv.cc:54-92: every (pixel, view)
  * keeps its own value, img[(iy*nx + ix)*na + ia] = the view's linear
  * interpolation at s (zero off the detector), with no sum over views and no

@@ -75,6 +75,8 @@ class _Lib:
                 args.append(_ptr(a))
             elif a is None:
                 args.append(None)
+            elif isinstance(a, C.c_double):
+                args.append(a)
             else:
                 args.append(C.c_int(int(a)))
         ok = self.fn(name)(*args)
@@ -195,6 +197,25 @@ def fp(g: Geom, img: np.ndarray, tables, kind="oracle"):
     proj = np.zeros((g.na, g.ns), img.dtype)
     sc, bo, usex = tables
     lib(kind).call("fp_" + _suffix(img.dtype), g, img, proj, sc, bo, usex)
+    return proj
+
+
+def fp_rc_prep(g: Geom, step=0.125, kind="oracle"):
+    """Ray-casting FP tables (src/fp_par_2d.cc:18-89): sincos [na][2],
+    entry points [na][ns][2], step counts [na][ns]."""
+    sc = np.zeros(2 * g.na); begins = np.zeros(2 * g.na * g.ns)
+    nsteps = np.zeros(g.na * g.ns, np.int32)
+    lib(kind).call("fp_rc_prep", g, C.c_double(step), sc, begins, nsteps)
+    return sc, begins, nsteps
+
+
+def fp_rc(g: Geom, img: np.ndarray, tables, step=0.125, kind="oracle"):
+    """Ray-casting parallel FP (src/fp_par_2d.cc:91-144)."""
+    img = np.ascontiguousarray(img)
+    proj = np.zeros((g.na, g.ns), img.dtype)
+    sc, begins, nsteps = tables
+    lib(kind).call("fp_rc_" + _suffix(img.dtype), g, C.c_double(step), img, proj, sc, begins,
+                   nsteps)
     return proj
 
 

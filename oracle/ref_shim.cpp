@@ -164,6 +164,26 @@ int ref_par_svbp_d(const orc_geom* g, const double* p, double* img, const double
         .calculate_on_cpu(p, img, xcos, ysin, nullptr);
 }
 
+int ref_fp_rc_prep(const orc_geom* g, double step, double* sc, double* begins, int* nsteps)
+{
+    return ParallelProjection2DRayCastingPrepare(fp_param(g), step)
+        .calculate_on_cpu(sc, begins, nsteps);
+}
+
+int ref_fp_rc_f(const orc_geom* g, double step, const float* img, float* proj, const double* sc,
+                const double* begins, const int* nsteps)
+{
+    return ParallelProjection2DRayCasting<float>(fp_param(g), step)
+        .calculate_on_cpu(img, proj, sc, begins, nsteps);
+}
+
+int ref_fp_rc_d(const orc_geom* g, double step, const double* img, double* proj,
+                const double* sc, const double* begins, const int* nsteps)
+{
+    return ParallelProjection2DRayCasting<double>(fp_param(g), step)
+        .calculate_on_cpu(img, proj, sc, begins, nsteps);
+}
+
 int ref_fp_prep(const orc_geom* g, double* sc, double* bo, int* usex)
 {
     return ParallelProjection2DRayDrivenPrepare(fp_param(g)).calculate_on_cpu(sc, bo, usex);

@@ -61,6 +61,10 @@ PAR_SMALL = dict(ns=41, na=36, ds=0.8, orbit=math.pi / 36, offset_s=-0.35,
 SVBP_SMALL = dict(ns=15, na=12, ds=0.8, orbit=math.pi / 12, offset_s=0.4,
                   orbit_start=0.2, nx=18, ny=14, dx=0.7, dy=0.9, offset_x=0.3,
                   offset_y=-0.2)
+# Ray-casting FP: a FOV circle wider than the image (samples past its edges)
+# and rays that miss the circle.
+RC_SMALL = dict(ns=61, na=48, ds=1.1, orbit=math.pi / 48, offset_s=0.3, orbit_start=0.05,
+                nx=40, ny=36, dx=1.0, dy=1.0, offset_x=0.2, offset_y=-0.3, fov=24.0)
 # Ray-driven FP: nx == ny (the reference's transpose requires it); the
 # detector covers the image so the reference's transpose stays defined.
 FP_SMALL = dict(ns=41, na=40, ds=1.0, orbit=math.pi / 40, offset_s=0.3,
@@ -124,6 +128,15 @@ def svbp_fixtures():
          svbp_f64=O.par_svbp(g, proj.astype(np.float64), t, kind="ref"))
 
 
+def rc_fixtures():
+    g = O.geom(**RC_SMALL)
+    t = O.fp_rc_prep(g, kind="ref")
+    img = randn(81, (g.ny, g.nx))
+    save("rc_small", g, img=img, sincostbl=t[0], begins=t[1], nsteps=t[2],
+         fp_f32=O.fp_rc(g, img, t, kind="ref"),
+         fp_f64=O.fp_rc(g, img.astype(np.float64), t, kind="ref"))
+
+
 def fp_fixtures():
     g = O.geom(**FP_SMALL)
     t = O.fp_prep(g, "ref")
@@ -176,5 +189,6 @@ if __name__ == "__main__":
     fan_fixtures()
     par_fixtures()
     svbp_fixtures()
+    rc_fixtures()
     fp_fixtures()
     filter_fixtures()

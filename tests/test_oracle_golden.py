@@ -73,6 +73,18 @@ def test_svbp_small():
     np.testing.assert_array_equal(O.par_svbp(g, d["proj"].astype(np.float64), t), d["svbp_f64"])
 
 
+def test_rc_small():
+    """Ray-casting FP against the compiled reference's outputs."""
+    geo, d = load_golden("rc_small")
+    g = G(geo)
+    t = O.fp_rc_prep(g)
+    np.testing.assert_array_equal(t[0], d["sincostbl"])
+    np.testing.assert_array_equal(t[1], d["begins"])
+    np.testing.assert_array_equal(t[2], d["nsteps"])
+    np.testing.assert_array_equal(O.fp_rc(g, d["img"], t), d["fp_f32"])
+    np.testing.assert_array_equal(O.fp_rc(g, d["img"].astype(np.float64), t), d["fp_f64"])
+
+
 def test_fp_small():
     geo, d = load_golden("fp_small")
     g = G(geo)

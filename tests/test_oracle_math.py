@@ -107,6 +107,21 @@ def test_svbp_views_sum_to_parallel_bp():
                                atol=1e-13)
 
 
+def test_raycasting_fp_of_constant_is_chord_length():
+    """Ray-casting FP of a constant c over an image that contains the FOV
+    circle: each ray is c times its chord 2 sqrt(fov^2 - s^2) within two
+    steps (the march stops at the last whole step), and 0 off the circle."""
+    step, c = 0.125, 1.5
+    g = O.geom(ns=41, na=24, ds=1.3, orbit=math.pi / 24, offset_s=0.4, orbit_start=0.1,
+               nx=64, ny=64, dx=1.0, dy=1.0, fov=24.0)
+    proj = O.fp_rc(g, np.full((g.ny, g.nx), c), O.fp_rc_prep(g, step), step)
+    s = (np.arange(g.ns) - ((g.ns - 1) / 2 + g.offset_s)) * g.ds
+    chord = 2 * np.sqrt(np.maximum(g.fov ** 2 - s ** 2, 0))
+    assert (chord > 0).sum() > 20 and (chord == 0).sum() >= 4
+    np.testing.assert_allclose(proj, np.broadcast_to(c * chord, proj.shape), atol=2 * c * step)
+    assert np.all(proj[:, chord == 0] == 0)
+
+
 def test_fan_bp_constant_sinogram_matches_source_distances():
     """BP of a constant c is c*scale*sum_beta dsd^2/|P - S(beta)|^2, with the
     source at S = (-dso sin b, dso cos b) computed here independently."""

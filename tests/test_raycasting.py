@@ -1,0 +1,65 @@
+"""Ray-casting parallel FP (src/fp_par_2d.cc:18-144, the TF FP op's
+method "raycasting") on the GPU: bit-identical to the compiled reference's
+outputs and to the oracle in float32 and float64."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.conftest import load_golden, randn
+
+
+def kwargs(geo):
+    return dict(img_shape=[geo["ny"], geo["nx"]], img_space=[geo["dy"], geo["dx"]],
+                proj_shape=[geo["na"], geo["ns"]], channel_space=geo["ds"],
+                img_offset=[geo["offset_y"], geo["offset_x"]], channel_offset=geo["offset_s"],
+                orbit_start=geo["orbit_start"], orbit=geo["orbit"], fov=geo["fov"],
+                method="raycasting")
+
+
+def test_raycasting_has_no_gradient_like_the_reference():
+    import ct_recon_extensions_amd as F
+    geo, _ = load_golden("rc_small")
+    with pytest.raises(RuntimeError, match="not yet implemented for raycasting"):
+        F.parallel_projection_2d_grad(torch.zeros(1), **kwargs(geo))
+
+
+@pytest.fixture(scope="module")
+def F():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import ct_recon_extensions_amd as F
+    return F
+
+
+@pytest.mark.gpu
+def test_raycasting_golden_bit_exact(F):
+    geo, d = load_golden("rc_small")
+    out = F.parallel_projection_2d(torch.from_numpy(d["img"]).cuda(), **kwargs(geo))
+    assert out.shape == (geo["na"], geo["ns"])
+    np.testing.assert_array_equal(out.cpu().numpy(), d["fp_f32"])
+    out64 = F.parallel_projection_2d(torch.from_numpy(d["img"]).double().cuda()[None, ..., None],
+                                     **kwargs(geo))
+    assert out64.shape == (1, geo["na"], geo["ns"], 1)
+    np.testing.assert_array_equal(out64[0, ..., 0].cpu().numpy(), d["fp_f64"])
+
+
+# A FOV circle wider than the image on one side (samples past the image's
+# edges skipped), rays missing it, non-square pixels, 257 rays per view
+# (a partial last block), three slices.
+GEO = dict(ns=257, na=30, ds=0.9, orbit=math.pi / 30, offset_s=-0.6, orbit_start=0.2, nx=90,
+           ny=70, dx=1.0, dy=1.2, offset_x=3.0, offset_y=-2.0, fov=48.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_raycasting_batch_bit_exact_vs_oracle(F, dtype):
+    g = O.geom(**GEO)
+    t = O.fp_rc_prep(g)
+    assert (t[2] == 0).sum() > 100 and (t[2] > 0).sum() > 1000
+    img = randn(91, (3, g.ny, g.nx), dtype)
+    out = F.parallel_projection_2d(torch.from_numpy(img).cuda(), **kwargs(GEO)).cpu().numpy()
+    for b in range(3):
+        np.testing.assert_array_equal(out[b], O.fp_rc(g, img[b], t), err_msg=f"slice {b}")
