@@ -152,6 +152,11 @@ def profile_kernels(torch, F, which):
         x = torch.randn((8, scan.n, scan.n), generator=g, device="cuda")
         for _ in range(3):
             img = fpt(fp(x))
+    elif which == "rc":
+        c = SVBP_CFG
+        x = torch.randn((1, c["n"], c["n"]), generator=g, device="cuda")
+        for _ in range(3):
+            img = F.parallel_projection_2d(x, **kw_rc())
     elif which == "svbp":
         c = SVBP_CFG
         proj = torch.randn((1, c["na"], c["ns"]), generator=g, device="cuda")
@@ -184,6 +189,52 @@ def kw_svbp():
     c = SVBP_CFG
     return dict(img_shape=[c["n"], c["n"]], img_space=[1.0, 1.0], proj_shape=[c["na"], c["ns"]],
                 channel_space=1.0, orbit=math.pi / c["na"])
+
+
+RC_SEED = 20261208
+# Ray-casting FP workload: the single-view BP's geometry with the FOV circle
+# inscribed in the image (fov = 256), the reference's step 0.125.
+PEAK_FP64_TFLOPS = 78.6
+RC_FLOPS_PER_SAMPLE = 16  # bilinear lerp 14 (u, v, 1-u, 1-v, 6 mul, 3 add, sum) + 2 advances
+
+
+def kw_rc():
+    return dict(kw_svbp(), fov=256.0, method="raycasting")
+
+
+def bench_raycasting(torch, F, reps, with_cpu=True):
+    """Ray-casting parallel FP (src/fp_par_2d.cc:18-144) of one seeded random
+    512^2 image: GPU time, samples/s and the FP64 rate of its bilinear
+    samples, and the reference CPU path on one thread for the same slice."""
+    from ct_recon_extensions_amd.geometry import ParallelProjection2DParam, fp_rc_tables
+    c = SVBP_CFG
+    n, na, ns = c["n"], c["na"], c["ns"]
+    g = torch.Generator(device="cuda").manual_seed(RC_SEED)
+    img = torch.randn((1, n, n), generator=g, device="cuda")
+    out = F.parallel_projection_2d(img, **kw_rc())
+    torch.cuda.synchronize()
+    ms = time_events(lambda: F.parallel_projection_2d(img, **kw_rc()), reps, torch)
+    p = ParallelProjection2DParam(ns, na, 1.0, math.pi / na, 0.0, 0.0, n, n, 1.0, 1.0, 0.0, 0.0,
+                                  256.0)
+    samples = int(fp_rc_tables(p)[2].astype(np.int64).sum())
+    tflops = samples * RC_FLOPS_PER_SAMPLE / (ms * 1e-3) / 1e12
+    res = {"workload": "1 x 512^2, parallel 720 views/180deg, 729 ch, ds=dx=1, fov 256, step "
+                       "0.125; seeded N(0,1) image", "seed": RC_SEED, "gpu_ms": round(ms, 4),
+           "samples": samples, "gsamples_per_s": round(samples / (ms * 1e-3) / 1e9, 2),
+           "fp64_tflops": round(tflops, 3), "fp64_frac": round(tflops / PEAK_FP64_TFLOPS, 4)}
+    if with_cpu:
+        from oracle import oracle as O
+        kind = "ref" if O.have_ref() else "oracle"
+        geo = O.geom(ns=ns, na=na, ds=1.0, orbit=math.pi / na, nx=n, ny=n, dx=1.0, dy=1.0,
+                     fov=256.0)
+        t = O.fp_rc_prep(geo, kind=kind)
+        host = img[0].cpu().numpy()
+        t0 = time.perf_counter()
+        ref = O.fp_rc(geo, host, t, kind=kind)
+        res.update(cpu_kind="reference" if kind == "ref" else "port",
+                   cpu_1thread_ms=round((time.perf_counter() - t0) * 1e3, 1),
+                   bit_identical=bool(np.array_equal(out[0].cpu().numpy(), ref)))
+    return res
 
 
 def bench_svbp(torch, F, reps, with_cpu=True):
@@ -421,15 +472,17 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the filter precision study")
     ap.add_argument("--no-c1", action="store_true", help="skip the 256^2 parallel FBP")
     ap.add_argument("--no-svbp", action="store_true", help="skip the single-view parallel BP")
+    ap.add_argument("--no-rc", action="store_true", help="skip the ray-casting FP")
     ap.add_argument("--profile-bp", action="store_true",
                     help="counter-collection mode: three fan BP launches of the workload, then a "
                          "256 MiB device copy from a cold source (calibrates FETCH/WRITE_SIZE)")
-    ap.add_argument("--profile", choices=("bp", "filter", "lsq", "svbp"), default=None,
+    ap.add_argument("--profile", choices=("bp", "filter", "lsq", "svbp", "rc"), default=None,
                     help="counter-collection mode for one kernel family: 'bp' as --profile-bp; "
                          "'filter' three fused fan-weighted ramp filters (default implementation) of the "
                          "workload; 'lsq' three FP and FP^T launches of the least-squares "
-                         "workload; 'svbp' three single-view parallel BPs of its workload; each "
-                         "followed by the calibration copy")
+                         "workload; 'svbp' three single-view parallel BPs of its workload; 'rc' "
+                         "three ray-casting FPs of its workload; each followed by the "
+                         "calibration copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path shards by slice with no exchange, so "
@@ -593,6 +646,11 @@ def main():
         sv = result["single_view_bp"]
         print(f"[bench] single-view BP {sv['gpu_ms']:.3f} ms, {sv['gb_s']:.0f} GB/s",
               file=sys.stderr, flush=True)
+    if not args.no_rc:
+        result["raycasting_fp"] = bench_raycasting(torch, F, 5, with_cpu=not args.no_cpu)
+        rc_ = result["raycasting_fp"]
+        print(f"[bench] ray-casting FP {rc_['gpu_ms']:.3f} ms, {rc_['gsamples_per_s']:.0f} "
+              f"Gsamples/s", file=sys.stderr, flush=True)
     if not args.no_precision:
         result["filter_precision"] = bench_filter_precision(torch, args.steps)
         fp_ = result["filter_precision"]
