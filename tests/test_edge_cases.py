@@ -52,6 +52,9 @@ def test_empty_batches(env):
     fkw = dict(proj_shape=[10, 21], channel_space=1.0, dso=100.0, dsd=180.0)
     assert F.fan_weighting(proj, **fkw).shape == (0, 10, 21, 1)
     assert F.fan_filter(proj, **fkw).shape == (0, 10, 21, 1)
+    assert F.parallel_single_view_bp_2d(proj, **pkw).shape == (0, 16, 12, 10)
+    assert F.parallel_projection_2d(img, **pkw, fov=8.0, method="raycasting").shape == \
+        (0, 10, 21, 1)
 
 
 @pytest.mark.parametrize("geo", [
@@ -91,3 +94,28 @@ def test_fp_and_par_bp_smallest_geometries(env, geo):
     proj = randn(6, (geo["na"], geo["ns"]))
     bp = F.parallel_backprojection_2d(torch.from_numpy(proj)[None].cuda(), **kw)[0].cpu().numpy()
     assert rel_l2(bp, O.par_bp(g, proj, O.par_prep(g))) < TOL32
+
+
+@pytest.mark.parametrize("geo", [
+    dict(ns=1, na=6, ds=1.0, orbit=math.pi / 6, nx=7, ny=5, dx=1.0, dy=1.0, fov=-1.0),
+    dict(ns=23, na=1, ds=0.9, orbit=math.pi, orbit_start=0.7, nx=9, ny=11, dx=1.1, dy=0.8,
+         fov=6.0),
+    dict(ns=15, na=9, ds=1.0, orbit=math.pi / 9, nx=1, ny=1, dx=2.0, dy=2.0, fov=3.0,
+         offset_s=0.2),
+], ids=["one_channel", "one_view", "one_pixel"])
+def test_single_view_bp_and_raycasting_smallest_geometries(env, geo):
+    """Bit-identical to the oracle on the smallest geometries (a single
+    channel, a single view, a 1 x 1 image; the default fov of -1 leaves only
+    the rays within one unit of the centre)."""
+    torch, F = env
+    g = O.geom(**geo)
+    kw = dict(img_shape=[geo["ny"], geo["nx"]], img_space=[geo["dy"], geo["dx"]],
+              proj_shape=[geo["na"], geo["ns"]], channel_space=geo["ds"], orbit=geo["orbit"],
+              orbit_start=geo.get("orbit_start", 0.0), channel_offset=geo.get("offset_s", 0.0))
+    proj = randn(7, (geo["na"], geo["ns"]))
+    sv = F.parallel_single_view_bp_2d(torch.from_numpy(proj).cuda(), **kw).cpu().numpy()
+    np.testing.assert_array_equal(sv, O.par_svbp(g, proj, O.par_prep(g)))
+    img = randn(8, (geo["ny"], geo["nx"]))
+    rc = F.parallel_projection_2d(torch.from_numpy(img).cuda(), **kw, fov=geo["fov"],
+                                  method="raycasting").cpu().numpy()
+    np.testing.assert_array_equal(rc, O.fp_rc(g, img, O.fp_rc_prep(g)))
