@@ -22,7 +22,10 @@ namespace ctr {
 
 namespace {
 
+// kVpt views per computing thread: 4 keeps the kernel at 64 VGPRs (8 waves
+// per SIMD); 8 needs 120 (4 waves per SIMD) and measured 7% slower.
 constexpr int kSvPix = 64, kSvViews = 32, kSvPad = kSvViews + 1;
+constexpr int kVpt = 4, kThreads = kSvPix * (kSvViews / kVpt), kStage = 1024 / kThreads;
 
 struct SvbpProblem {
     int ns, na, npix;
@@ -34,7 +37,7 @@ struct SvbpProblem {
 // (na doubles of table, na values of output) and the buffers are 16-byte
 // aligned; otherwise element by element.
 template <typename T>
-__global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ proj,
+__global__ __launch_bounds__(kThreads) void par_svbp_kernel(const T* __restrict__ proj,
                                                        T* __restrict__ img,
                                                        const double* __restrict__ xcos,
                                                        const double* __restrict__ ysin,
@@ -49,29 +52,29 @@ __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ pro
     const int b = blockIdx.y;
     const T* pr = proj + static_cast<size_t>(b) * P.na * P.ns;
     T* out = img + (static_cast<size_t>(b) * P.npix + pix0) * P.na;
-    // staging: element e = tid + 256 k (k < 4) of [64 pixels][16 pairs of
-    // views]: pixel e >> 4, views 2 (e & 15) and 2 (e & 15) + 1
-    const double* xc[4];
-    const double* ys[4];
-    bool s_ok[4];
+    // staging: element e = tid + kThreads k (k < kStage) of [64 pixels][16
+    // pairs of views]: pixel e >> 4, views 2 (e & 15) and 2 (e & 15) + 1
+    const double* xc[kStage];
+    const double* ys[kStage];
+    bool s_ok[kStage];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int pix = pix0 + ((tid + 256 * k) >> 4);
+    for (int k = 0; k < kStage; ++k) {
+        const int pix = pix0 + ((tid + kThreads * k) >> 4);
         s_ok[k] = pix < P.npix;
         const int iy = s_ok[k] ? pix / nx : 0, ix = s_ok[k] ? pix - iy * nx : 0;
         xc[k] = xcos + static_cast<size_t>(ix) * P.na;
         ys[k] = ysin + static_cast<size_t>(iy) * P.na;
     }
-    // computing: thread -> (pixel tid & 63, views 8 (tid >> 6) .. +7)
-    const int cp = tid & (kSvPix - 1), cv = 8 * (tid >> 6);
+    // computing: thread -> (pixel tid & 63, views kVpt (tid >> 6) .. + kVpt - 1)
+    const int cp = tid & (kSvPix - 1), cv = kVpt * (tid >> 6);
     const bool c_ok = pix0 + cp < P.npix;
 
     for (int va = 0; va < P.na; va += kSvViews) {
         const int nv = min(kSvViews, P.na - va);
         __syncthreads();  // the previous chunk's results are written out
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int e = tid + 256 * k, sp = e >> 4, v = 2 * (e & 15);
+        for (int k = 0; k < kStage; ++k) {
+            const int e = tid + kThreads * k, sp = e >> 4, v = 2 * (e & 15);
             if (!s_ok[k] || v >= nv) continue;
             if (vec && v + 1 < nv) {
                 const double2 x2 = *reinterpret_cast<const double2*>(xc[k] + va + v);
@@ -85,13 +88,13 @@ __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ pro
         }
         __syncthreads();
         if (c_ok) {
-            // positions and taps of all 8 views first, then the 16 gathers
-            // back to back (an unused view reads channel 0 of a valid row)
-            double u[8];
-            int i1[8], i2[8];
-            bool ok[8];
+            // positions and taps of the thread's views first, then their
+            // gathers back to back (an unused view reads channel 0 of a valid row)
+            double u[kVpt];
+            int i1[kVpt], i2[kVpt];
+            bool ok[kVpt];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < kVpt; ++j) {
                 const int v = min(cv + j, nv - 1);
                 const double s = pos[cp][v] / P.ds + P.cents;
                 ok[j] = cv + j < nv && s >= 0 && s <= P.top;
@@ -100,15 +103,15 @@ __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ pro
                 i2[j] = ok[j] ? static_cast<int>(ceil(s)) : 0;
                 u[j] = s - f;
             }
-            T p0[8], p1[8];
+            T p0[kVpt], p1[kVpt];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < kVpt; ++j) {
                 const T* row = pr + static_cast<size_t>(va + min(cv + j, nv - 1)) * P.ns;
                 p0[j] = row[i1[j]];
                 p1[j] = row[i2[j]];
             }
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < kVpt; ++j) {
                 const T r = ok[j] ? static_cast<T>((1 - u[j]) * static_cast<double>(p0[j]) +
                                                    u[j] * static_cast<double>(p1[j]))
                                   : T(0);
@@ -118,7 +121,7 @@ __global__ __launch_bounds__(256) void par_svbp_kernel(const T* __restrict__ pro
         __syncthreads();
         // writing: element e of [64 pixels][32 / kTv pieces of 16 bytes]
         constexpr int kPieces = kSvViews / kTv;
-        for (int e = tid; e < kSvPix * kPieces; e += 256) {
+        for (int e = tid; e < kSvPix * kPieces; e += kThreads) {
             const int wp = e / kPieces, v = kTv * (e - wp * kPieces);
             if (pix0 + wp >= P.npix || v >= nv) continue;
             T* o = out + static_cast<size_t>(wp) * P.na + va + v;
@@ -161,11 +164,11 @@ int par_svbp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbat
     const int vec = (g.na * 8) % 16 == 0 && (g.na * esize) % 16 == 0 && a16(img) && a16(xcos) &&
                     a16(ysin);
     if (dtype == CTR_F64)
-        hipLaunchKernelGGL(par_svbp_kernel<double>, grid, dim3(256), 0, st,
+        hipLaunchKernelGGL(par_svbp_kernel<double>, grid, dim3(kThreads), 0, st,
                            static_cast<const double*>(proj), static_cast<double*>(img), xcos,
                            ysin, P, static_cast<int>(g.nx), vec);
     else
-        hipLaunchKernelGGL(par_svbp_kernel<float>, grid, dim3(256), 0, st,
+        hipLaunchKernelGGL(par_svbp_kernel<float>, grid, dim3(kThreads), 0, st,
                            static_cast<const float*>(proj), static_cast<float*>(img), xcos, ysin,
                            P, static_cast<int>(g.nx), vec);
     return check_launch("par_svbp_kernel");
