@@ -4,16 +4,29 @@
 // a sample inside [0, nx-1] x [0, ny-1] adds the bilinear interpolation of
 // the image, summed in FP64 in ray order, and the ray is sum * step * dx.
 //
-// One thread per (ray, slice), rays of a view on consecutive threads (their
-// samples walk parallel lines, so a wave's four gathers per step touch
-// neighbouring pixels).  The arithmetic is the reference's, in FP64 without
-// contraction and in the same order, rounded to T once: the result matches
-// it bit for bit.
+// A wave holds 8 adjacent rays x 8 step phases: lane j of a ray samples the
+// ray's steps j, j + 8, j + 16, ..., so one gather instruction covers a
+// patch of 8 rays x 8 steps (8 x 1 pixels).  With one ray per lane, a
+// wave's 64 samples of one step lie on a segment across the rays that at
+// steep angles crosses one image row per lane (53 cache lines per gather
+// measured, which saturated the vector L1); the patch needs a few.
+//
+// The arithmetic is the reference's, in FP64 without contraction and in the
+// same order, rounded to T once, so the result matches it bit for bit:
+// - every lane advances its ray one step at a time, as the reference
+//   accumulates x += step sin;
+// - the 8 lanes of a ray sum the ray's samples in step order with width-8
+//   shuffles, each lane computing the same sum;
+// - a step past the ray's end or outside the image contributes +0.  The sum
+//   starts at +0 and never becomes -0 under round-to-nearest, so adding +0
+//   leaves it unchanged.
 #include "ctr_common.hpp"
 
 namespace ctr {
 
 namespace {
+
+constexpr int kRcPhases = 8, kRcRays = 256 / kRcPhases;  // per 256-thread block
 
 struct RcProblem {
     int ns, na, nx, ny, nrays;
@@ -29,19 +42,24 @@ __global__ __launch_bounds__(256) void fp_rc_kernel(const T* __restrict__ img,
                                                     const RcProblem P)
 {
 #pragma clang fp contract(off)
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= P.nrays) return;
+    const int j = threadIdx.x % kRcPhases;
+    const int r = blockIdx.x * kRcRays + threadIdx.x / kRcPhases;
+    const bool valid = r < P.nrays;
     const int b = blockIdx.y;
-    const int a = r / P.ns;
     const T* im = img + static_cast<size_t>(b) * P.nx * P.ny;
-    const int n = nsteps[r];
-    T out = T(0);
+    const int n = valid ? nsteps[r] : 0;
     if (n > 0) {
+        const int a = r / P.ns;
         const double stx = P.step * sincostbl[2 * a], sty = P.step * sincostbl[2 * a + 1];
         double x = begins[2 * static_cast<size_t>(r)], y = begins[2 * static_cast<size_t>(r) + 1];
+        for (int t = 0; t < j; ++t) {
+            x += stx;
+            y += sty;
+        }
         double sum = 0;
-        for (int k = 0; k < n; ++k) {
-            if (x >= 0 && x <= P.xmax && y >= 0 && y <= P.ymax) {
+        for (int k0 = 0; k0 < n; k0 += kRcPhases) {
+            double val = 0;
+            if (k0 + j < n && x >= 0 && x <= P.xmax && y >= 0 && y <= P.ymax) {
                 const double fx = floor(x), fy = floor(y);
                 const int ix1 = static_cast<int>(fx), ix2 = static_cast<int>(ceil(x));
                 const int iy1 = static_cast<int>(fy), iy2 = static_cast<int>(ceil(y));
@@ -50,14 +68,20 @@ __global__ __launch_bounds__(256) void fp_rc_kernel(const T* __restrict__ img,
                 const double p21 = static_cast<double>(im[ix2 + iy1 * P.nx]);
                 const double p12 = static_cast<double>(im[ix1 + iy2 * P.nx]);
                 const double p22 = static_cast<double>(im[ix2 + iy2 * P.nx]);
-                sum += (1 - v) * ((1 - u) * p11 + u * p21) + v * ((1 - u) * p12 + u * p22);
+                val = (1 - v) * ((1 - u) * p11 + u * p21) + v * ((1 - u) * p12 + u * p22);
             }
-            x += stx;
-            y += sty;
+#pragma unroll
+            for (int t = 0; t < kRcPhases; ++t) sum += __shfl(val, t, kRcPhases);
+#pragma unroll
+            for (int t = 0; t < kRcPhases; ++t) {
+                x += stx;
+                y += sty;
+            }
         }
-        out = static_cast<T>(sum * P.step * P.dx);
+        if (j == 0) proj[static_cast<size_t>(b) * P.nrays + r] = static_cast<T>(sum * P.step * P.dx);
+    } else if (valid && j == 0) {
+        proj[static_cast<size_t>(b) * P.nrays + r] = T(0);
     }
-    proj[static_cast<size_t>(b) * P.nrays + r] = out;
 }
 
 }  // namespace
@@ -82,7 +106,7 @@ int fp_rc(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
     P.dx = g.dx;
     P.xmax = static_cast<double>(g.nx) - 1;
     P.ymax = static_cast<double>(g.ny) - 1;
-    const dim3 grid(static_cast<unsigned int>((nrays + 255) / 256), nbatch);
+    const dim3 grid(static_cast<unsigned int>((nrays + kRcRays - 1) / kRcRays), nbatch);
     if (dtype == CTR_F64)
         hipLaunchKernelGGL(fp_rc_kernel<double>, grid, dim3(256), 0, st,
                            static_cast<const double*>(img), static_cast<double*>(proj), sincostbl,
