@@ -63,3 +63,23 @@ def test_raycasting_batch_bit_exact_vs_oracle(F, dtype):
     out = F.parallel_projection_2d(torch.from_numpy(img).cuda(), **kwargs(GEO)).cpu().numpy()
     for b in range(3):
         np.testing.assert_array_equal(out[b], O.fp_rc(g, img[b], t), err_msg=f"slice {b}")
+
+
+# A FOV circle just wider than the outermost channels: those rays take 4
+# steps, the others 42-64 (mostly not multiples of the kernel's 8 step
+# phases); 63 rays, so the last 32-ray block is partial.
+GEO_SHORT = dict(ns=9, na=7, ds=1.0, orbit=math.pi / 7, offset_s=0.0, orbit_start=0.0, nx=12,
+                 ny=12, dx=1.0, dy=1.0, offset_x=0.0, offset_y=0.0, fov=4.01)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_raycasting_short_and_ragged_rays_bit_exact(F, dtype):
+    g = O.geom(**GEO_SHORT)
+    t = O.fp_rc_prep(g)
+    n = t[2]
+    assert ((n > 0) & (n < 8)).sum() >= 14 and ((n % 8) != 0).sum() > 40 and (n == 64).any()
+    img = randn(92, (2, g.ny, g.nx), dtype)
+    out = F.parallel_projection_2d(torch.from_numpy(img).cuda(), **kwargs(GEO_SHORT)).cpu().numpy()
+    for b in range(2):
+        np.testing.assert_array_equal(out[b], O.fp_rc(g, img[b], t), err_msg=f"slice {b}")
