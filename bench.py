@@ -162,6 +162,24 @@ def profile_kernels(torch, F, which):
         proj = torch.randn((1, c["na"], c["ns"]), generator=g, device="cuda")
         for _ in range(3):
             img = F.parallel_single_view_bp_2d(proj, **kw_svbp())
+    elif which == "bpt":
+        x = torch.randn((cfg["nbatch"], cfg["n"], cfg["n"]), generator=g, device="cuda")
+        for det in (False, True):
+            for _ in range(3):
+                img = F.fan_backprojection_2d_grad(x, **kw_fan(cfg), deterministic=det)
+    elif which == "ragged":
+        from ct_recon_extensions_amd import ragged as R
+        params, angles = R.random_fan_problems(64, seed=RAGGED_SEED)
+        rb = R.RaggedFanBackprojection2DAngle(params, angles, tile=0, vchunk=32)
+        rb.proj_flat.normal_(generator=g)
+        for _ in range(3):
+            rb(graph=False)
+        img = rb.img_flat
+    elif which == "filter5":
+        from ct_recon_extensions_amd import phantom as P
+        x, kw, _ = P.shepp_logan_fan_batch(32, PRECISION_SEED, "cuda")
+        for _ in range(3):
+            img = F.fan_filter(x, **kw, impl="bf16x3")
     else:
         proj = torch.randn((cfg["nbatch"], cfg["na"], cfg["ns"]), generator=g, device="cuda")
         for _ in range(3):
@@ -360,7 +378,20 @@ def bench_lsq(torch, steps):
         "rel_err_last": float((x - xt).norm() / xt.norm()),
         "fp_ray_samples_per_s": 8 * scan.na * scan.ns * scan.n / (ms_fp * 1e-3),
         "fpt_pixel_views_per_s": 8 * scan.na * scan.n ** 2 / (ms_fpt * 1e-3),
+        "roofline_fp": _valu_roofline("fp_lds_kernel", 8 * scan.na * scan.ns * scan.n,
+                                      FP_SLOTS_PER_SAMPLE, ms_fp, "ray sample"),
+        "roofline_fpt": _valu_roofline("bp_forward_kernel<FptGeo> + border passes",
+                                       8 * scan.na * scan.n ** 2, FPT_SLOTS_PER_PV, ms_fpt,
+                                       "pixel*view"),
     }
+
+
+def _valu_roofline(kernel, units, slots_per_unit, ms, unit_name):
+    achieved = units * slots_per_unit / (ms * 1e-3)
+    return {"kernel": kernel, "bound": "valu", "achieved": round(achieved / 1e12, 3),
+            "peak": round(PEAK_VALU_SLOTS / 1e12, 3), "unit": "Tslot/s (FP32 VALU lane-issues)",
+            "frac": round(achieved / PEAK_VALU_SLOTS, 4),
+            "model": f"{slots_per_unit} slots per {unit_name} x {units:.4g} per launch"}
 
 
 def bench_ragged(torch, reps):
@@ -386,6 +417,9 @@ def bench_ragged(torch, reps):
         "work_items": rb.nitems, "grid": rb.grid, "busy_fraction": round(busy, 4),
         "unbatched_28slot_model_frac": round(
             rb.pixel_views / (ms * 1e-3) * UNBATCHED_FAN_SLOTS / PEAK_VALU_SLOTS, 4),
+        # one slice per problem: the geometry of every pixel*view is its own
+        "roofline": _valu_roofline("ragged_fan_bp_kernel", rb.pixel_views,
+                                   FAN_GEOM_SLOTS + FAN_SLICE_SLOTS, ms, "pixel*view"),
     }
     rb.close()
     return out
@@ -415,14 +449,184 @@ def bench_filter_precision(torch, reps):
                                             torch), 4)
     # Parity-split Toeplitz GEMM: M = rows, N = K = ns, half the dense MACs.
     macs = nb * na * ns * ns / 2
-    res["roofline_bf16x3"] = {"bound": "mfma", "flops": 3 * 2 * macs,
-                              "achieved_tflops": round(3 * 2 * macs / (res["ms"]["bf16x3"] * 1e9), 2),
-                              "peak_tflops": 2500.0}
-    res["roofline_fp32_mfma"] = {"bound": "mfma", "flops": 2 * macs,
-                                 "achieved_tflops": round(2 * macs / (res["ms"]["mfma"] * 1e9), 2),
-                                 "peak_tflops": 157.3}
-    res["hbm_bytes"] = 2 * x.numel() * 4
+    hbm = 2 * x.numel() * 4
+    for impl, kern, mult, peak in (("bf16x3", "ramp_bf16x3_kernel", 3, PEAK_MFMA_BF16_TFLOPS),
+                                   ("mfma", "ramp_mfma_kernel", 1, PEAK_FP32_MFMA_TFLOPS)):
+        tf = mult * 2 * macs / (res["ms"][impl] * 1e9)
+        res["roofline_" + ("fp32_mfma" if impl == "mfma" else impl)] = {
+            "kernel": kern, "bound": "mfma", "achieved": round(tf, 2), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(tf / peak, 4), "flops": mult * 2 * macs,
+            "hbm": {"achieved": round(hbm / (res["ms"][impl] * 1e6), 1), "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": round(hbm / (res["ms"][impl] * 1e6) / PEAK_HBM_GBS, 4),
+                    "algorithmic_bytes": hbm}}
+    res["hbm_bytes"] = hbm
+    res["_x"] = (x, kw)
     return res
+
+
+BWD_SEED = 20261209
+# Algorithmic VALU issue slots of the ray-driven FP and of its transpose
+# (per ray sample and per pixel*view): 7 and 12, counting the interpolation,
+# the position step and the accumulation of one sample, as the per-unit
+# model of these two kernels.
+FP_SLOTS_PER_SAMPLE = 7
+FPT_SLOTS_PER_PV = 12
+PEAK_MFMA_BF16_TFLOPS = 2500.0
+PEAK_FP32_MFMA_TFLOPS = 157.3
+
+
+def bench_fbp_backward(torch, F, proj, reps):
+    """The differentiable fan FBP on the 16-slice workload: fan weighting +
+    ramp filter + bp_fan_2d forward and its backward through autograd (the
+    BP transpose, then the filter and the weighting transposed), per step;
+    the BP transpose alone in both modes (LDS float scatter, and the
+    order-independent fixed-point mode); and the adjoint gap of the pair
+    on seeded Gaussian inputs."""
+    cfg = CFG
+    B, n, na, ns = cfg["nbatch"], cfg["n"], cfg["na"], cfg["ns"]
+    kwf, kwb = kw_filter(cfg), kw_fan(cfg)
+    g = torch.Generator(device="cuda").manual_seed(BWD_SEED)
+    gimg = torch.randn((B, n, n), generator=g, device="cuda")
+    x = proj.detach().clone().requires_grad_(True)
+
+    def fwd_bwd():
+        x.grad = None
+        F.fan_backprojection_2d(F.fan_filter(x, **kwf), **kwb).backward(gimg)
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    ms_fb = time_events(fwd_bwd, reps, torch)
+    ms_bpt = time_events(lambda: F.fan_backprojection_2d_grad(gimg, **kwb), reps, torch)
+    ms_bpt_det = time_events(
+        lambda: F.fan_backprojection_2d_grad(gimg, **kwb, deterministic=True), reps, torch)
+    pr = torch.randn((B, na, ns), generator=g, device="cuda")
+    ap = F.fan_backprojection_2d(pr, **kwb).double()
+    aty = F.fan_backprojection_2d_grad(gimg, **kwb).double()
+    gap = abs(float((ap * gimg.double()).sum()) - float((pr.double() * aty).sum()))
+    gap /= float(ap.norm()) * float(gimg.double().norm())
+    a1 = F.fan_backprojection_2d_grad(gimg, **kwb, deterministic=True)
+    a2 = F.fan_backprojection_2d_grad(gimg, **kwb, deterministic=True)
+    pv = B * n * n * na
+    groups = math.ceil(B / 8)  # the scatter kernel carries 8 slices per workgroup
+    slots = n * n * na * FAN_GEOM_SLOTS * groups + pv * FAN_SLICE_SLOTS
+    return {
+        "workload": "16 x 512^2 fan FBP (720 views, 736 ch): forward + backward via autograd; "
+                    "backward = BP^T, ramp filter, fan weighting",
+        "seed": BWD_SEED, "ms_fwd_bwd_step": round(ms_fb, 4),
+        "bpt_ms": round(ms_bpt, 4), "bpt_det_ms": round(ms_bpt_det, 4),
+        "bpt_gpixel_views_per_s": round(pv / (ms_bpt * 1e-3) / 1e9, 1),
+        "bpt_det_gpixel_views_per_s": round(pv / (ms_bpt_det * 1e-3) / 1e9, 1),
+        "bpt_valu_slot_model_frac": round(slots / (ms_bpt * 1e-3) / PEAK_VALU_SLOTS, 4),
+        "adjoint_gap": gap,
+        "det_bitwise_repeatable": bool(torch.equal(a1, a2)),
+    }
+
+
+def _cpu_kind():
+    from oracle import oracle as O
+    return ("ref", "reference") if O.have_ref() else ("oracle", "port")
+
+
+def cpu_lsq(threads):
+    """The reference CPU path of one least-squares gradient step's operators,
+    ray-driven FP then its transpose (src/fp_par_2d.cc:210-391), on one
+    512^2 slice of the workload: on one thread, and one slice per thread on
+    `threads` threads; reported per 8-slice step."""
+    from ct_recon_extensions_amd import phantom as P
+    from oracle import oracle as O
+    kind, label = _cpu_kind()
+    n, na, ns = 512, 720, 729
+    g = O.geom(ns=ns, na=na, ds=1.0, orbit=math.pi / na, nx=n, ny=n, dx=1.0, dy=1.0)
+    tf, tg = O.fp_prep(g, kind), O.fp_grad_prep(g, kind)
+    x = P.rasterize(P.random_ellipses(10, LSQ_SEED, radius=n / 2), n, n, 1.0, 1.0)
+    x = x.astype(np.float32)
+
+    def one(v):
+        return O.fp_grad(g, O.fp(g, v, tf, kind=kind), tg, kind=kind)
+
+    t0 = time.perf_counter()
+    one(x)
+    single = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, [x] * threads))
+    wall = time.perf_counter() - t0
+    return {"value": round(8 * wall / threads * 1e3, 1), "unit": "ms per 8-slice FP + FP^T",
+            "cores": threads, "kind": label,
+            "sample": f"{threads} slices of 512^2 x 720 views x 729 ch, FP then FP^T, one per "
+                      f"thread, {wall:.1f}s wall",
+            "single_thread_value": round(8 * single * 1e3, 1),
+            "single_thread_sample": f"1 slice on 1 thread, {single:.1f}s"}
+
+
+def cpu_ragged(threads):
+    """The reference CPU bp_fan_2d_angle (src/bp_fan_2d_angle.cc:47-91) on
+    the first `threads` problems of the seeded ragged batch, one per thread,
+    and on problem 0 alone on one thread; the whole batch's time is
+    extrapolated from the all-thread rate."""
+    from ct_recon_extensions_amd import ragged as R
+    from oracle import oracle as O
+    kind, label = _cpu_kind()
+    params, angles = R.random_fan_problems(64, seed=RAGGED_SEED)
+    rng = np.random.default_rng(RAGGED_SEED)
+    jobs = []
+    for p, a in list(zip(params, angles))[:threads]:
+        g = O.geom(ns=p.ns, na=p.na, ds=p.ds, offset_s=p.offset_s, nx=p.nx, ny=p.ny, dx=p.dx,
+                   dy=p.dy, offset_x=p.offset_x, offset_y=p.offset_y, dso=p.dso, dsd=p.dsd)
+        jobs.append((g, O.fan_angle_prep(g, a, kind),
+                     rng.standard_normal((p.na, p.ns)).astype(np.float32), p.na * p.nx * p.ny))
+
+    def one(job):
+        g, t, pr, _ = job
+        return O.fan_bp(g, pr, t, scaled=False, kind=kind)
+
+    t0 = time.perf_counter()
+    one(jobs[0])
+    single = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, jobs))
+    wall = time.perf_counter() - t0
+    pv = sum(j[3] for j in jobs)
+    total = sum(p.na * p.nx * p.ny for p in params)
+    rate = pv / wall
+    return {"value": round(rate / 1e9, 4), "unit": "Gpixel*view/s", "cores": threads,
+            "kind": label,
+            "sample": f"problems 0..{threads - 1} of the batch ({pv:.3g} pixel*views), one per "
+                      f"thread, {wall:.1f}s wall",
+            "batch_ms_extrapolated": round(total / rate * 1e3, 1),
+            "single_thread_value": round(jobs[0][3] / single / 1e9, 4),
+            "single_thread_sample": f"problem 0 on 1 thread, {single:.1f}s"}
+
+
+def cpu_filter(threads, x, kw, rows=360):
+    """The reference CPU fan weighting + ramp filter in FP64
+    (src/fan_weighting.cc:19-35, src/filter.cc:67-91, every one of the 2ns+1
+    taps per output as it loops) on `rows` views of one slice per thread, and
+    on one thread; reported per 32 x 1440 x 1472 batch."""
+    from oracle import oracle as O
+    kind, label = _cpu_kind()
+    nb, na, ns = x.shape
+    g = O.geom(ns=ns, nrow=rows, ds=kw["channel_space"], dso=kw["dso"], dsd=kw["dsd"], type=1)
+    taps = O.ramp_prep(g, kind)
+    host = [x[b, :rows].double().cpu().numpy() for b in range(min(threads, nb))]
+
+    def one(v):
+        return O.ramp(g, O.fanw(g, v, kind=kind), taps, kind=kind)
+
+    t0 = time.perf_counter()
+    one(host[0])
+    single = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, [host[i % len(host)] for i in range(threads)]))
+    wall = time.perf_counter() - t0
+    per_batch = nb * na / rows
+    return {"value": round(wall / threads * per_batch * 1e3, 1),
+            "unit": "ms per 32 x 1440 x 1472 batch (FP64)", "cores": threads, "kind": label,
+            "sample": f"{rows} rows of one slice per thread on {threads} threads, {wall:.1f}s wall",
+            "single_thread_value": round(single * per_batch * 1e3, 1),
+            "single_thread_sample": f"{rows} rows on 1 thread, {single:.1f}s"}
 
 
 def device_spec():
@@ -473,15 +677,22 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 256^2 parallel FBP")
     ap.add_argument("--no-svbp", action="store_true", help="skip the single-view parallel BP")
     ap.add_argument("--no-rc", action="store_true", help="skip the ray-casting FP")
+    ap.add_argument("--no-bwd", action="store_true",
+                    help="skip the differentiable FBP's backward (BP transpose)")
     ap.add_argument("--profile-bp", action="store_true",
                     help="counter-collection mode: three fan BP launches of the workload, then a "
                          "256 MiB device copy from a cold source (calibrates FETCH/WRITE_SIZE)")
-    ap.add_argument("--profile", choices=("bp", "filter", "lsq", "svbp", "rc"), default=None,
+    ap.add_argument("--profile", choices=("bp", "filter", "lsq", "svbp", "rc", "bpt", "ragged",
+                                          "filter5"), default=None,
                     help="counter-collection mode for one kernel family: 'bp' as --profile-bp; "
                          "'filter' three fused fan-weighted ramp filters (default implementation) of the "
                          "workload; 'lsq' three FP and FP^T launches of the least-squares "
                          "workload; 'svbp' three single-view parallel BPs of its workload; 'rc' "
-                         "three ray-casting FPs of its workload; each followed by the "
+                         "three ray-casting FPs of its workload; 'bpt' three fan BP "
+                         "transposes of the workload in each mode (scatter, then "
+                         "deterministic); 'ragged' three eager runs of the ragged batch "
+                         "(16x16 tiles, 32 views per round); 'filter5' three BF16x3 filters "
+                         "of the 32 x 1440 x 1472 precision workload; each followed by the "
                          "calibration copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -581,8 +792,23 @@ def main():
         torch.cuda.synchronize()
         ms_plain_fbp = (time.perf_counter() - t0) * 1e3
         plain_fbp_err = float((out_plain - img).double().norm() / out_plain.double().norm())
+        # the reference's own launch geometry: at most 32 workgroups of 512
+        # threads, grid-stride (cuda/cuda_common.h:30-38), one launch per slice
+        def plain_bp_refgrid():
+            _lib.check(lib.ctr_plain_fan_bp_grid(
+                C.byref(g), 0, C.c_void_p(filt.data_ptr()), C.c_void_p(out_plain.data_ptr()), B,
+                C.c_void_p(xpos.data_ptr()), C.c_void_p(ypos.data_ptr()),
+                C.c_void_p(sc.data_ptr()), p.scale, 32, 512, st), "plain reference grid")
+
+        ms_plain_ref = time_events(plain_bp_refgrid, 1, torch)
+        plain_ref_err = float((out_plain - img).double().norm() / out_plain.double().norm())
         plain = dict(ms_per_batch_bp=ms_plain, slices_per_s_bp=B / (ms_plain * 1e-3),
                      speedup_bp=ms_plain / ms_bp, rel_l2_fast_vs_plain=plain_err,
+                     reference_grid={"grid": "min(32, ceil(npix/512)) x 512 threads, grid-stride",
+                                     "ms_per_batch_bp": round(ms_plain_ref, 3),
+                                     "slices_per_s_bp": round(B / (ms_plain_ref * 1e-3), 1),
+                                     "speedup_bp": round(ms_plain_ref / ms_bp, 1),
+                                     "rel_l2_fast_vs_plain": plain_ref_err},
                      ms_per_batch_fbp=ms_plain_fbp, slices_per_s_fbp=B / (ms_plain_fbp * 1e-3),
                      speedup_fbp_slices_per_s=ms_plain_fbp / ms_step,
                      rel_l2_fast_vs_plain_fbp=plain_fbp_err)
@@ -627,14 +853,27 @@ def main():
         },
         "plain_per_slice_gpu": plain,
     }
+    cpu_threads = max(2, args.cpu_threads or min(16, os.cpu_count() or 1))
+    if not args.no_bwd:
+        result["fbp_backward"] = bench_fbp_backward(torch, F, proj, args.steps)
+        fb = result["fbp_backward"]
+        print(f"[bench] FBP forward+backward {fb['ms_fwd_bwd_step']:.3f} ms, BP^T "
+              f"{fb['bpt_ms']:.3f} ms ({fb['bpt_det_ms']:.3f} deterministic)", file=sys.stderr,
+              flush=True)
     if not args.no_lsq:
         result["lsq_grad_step"] = bench_lsq(torch, args.lsq_steps)
         print(f"[bench] lsq grad step {result['lsq_grad_step']['ms_per_grad_step']:.3f} ms",
               file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            result["lsq_grad_step"]["cpu_baseline"] = cpu_lsq(cpu_threads)
+            print("[bench] lsq CPU baseline done", file=sys.stderr, flush=True)
     if not args.no_ragged:
         result["ragged"] = bench_ragged(torch, args.steps)
         print(f"[bench] ragged {result['ragged']['ms_per_batch']:.3f} ms", file=sys.stderr,
               flush=True)
+        if not args.no_cpu:
+            result["ragged"]["cpu_baseline"] = cpu_ragged(cpu_threads)
+            print("[bench] ragged CPU baseline done", file=sys.stderr, flush=True)
 
     if not args.no_c1:
         result["c1_parallel_fbp"] = bench_c1(torch, F, args.cpu_threads or min(16, os.cpu_count() or 1),
@@ -654,8 +893,13 @@ def main():
     if not args.no_precision:
         result["filter_precision"] = bench_filter_precision(torch, args.steps)
         fp_ = result["filter_precision"]
+        x5, kw5 = fp_.pop("_x")
         print(f"[bench] filter bf16x3 {fp_['ms']['bf16x3']:.3f} ms, worst slice rel L2 "
               f"{fp_['worst_rel_l2']['bf16x3']:.2e}", file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            fp_["cpu_baseline"] = cpu_filter(cpu_threads, x5, kw5)
+            print("[bench] filter CPU baseline done", file=sys.stderr, flush=True)
+        del x5
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -41,6 +41,12 @@ class CtrRaggedProblem(C.Structure):
     ]
 
 
+class CtrFanBpLaunch(C.Structure):
+    """Mirror of ``ctr_fan_bp_launch`` (include/ctr_api.h)."""
+
+    _fields_ = [("tile", C.c_int), ("vchunk", C.c_int), ("nb", C.c_int)]
+
+
 class CtrError(RuntimeError):
     pass
 
@@ -62,9 +68,12 @@ _SIGS = {
     "ctr_fp_grad_tables": (C.c_int, [_G, _P, _P, _P, C.c_uint]),
     "ctr_ramp_taps": (C.c_int, [_G, _P]),
     "ctr_fan_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, _P]),
-    "ctr_fan_bp_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "ctr_fan_bp_ex": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double,
+                                C.POINTER(CtrFanBpLaunch), _P]),
     "ctr_fan_bp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, C.c_int, _P]),
     "ctr_plain_fan_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, _P]),
+    "ctr_plain_fan_bp_grid": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double,
+                                        C.c_int, C.c_int, _P]),
     "ctr_par_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
     "ctr_par_bp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P]),
     "ctr_par_sv_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P]),

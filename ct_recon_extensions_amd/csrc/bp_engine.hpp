@@ -483,7 +483,7 @@ struct BptLaunch {
     typename Geo::Problem P;
     int nbatch, tiles_x, tiles_y;
     double scale;
-    const double* fix_scale;  // DET: device pointer to 2^E
+    const double* fix_scale;  // DET: device array of the per-slice scales 2^E
 };
 
 template <class Geo, int TW, int TH, int NB, bool DET>
@@ -506,8 +506,14 @@ __global__ __launch_bounds__(256) void bp_transpose_kernel(const float* __restri
     const int b0 = blockIdx.y * NB;
     const int nb = min(NB, L.nbatch - b0);
     const int rlen = P.ns + 1;  // pad slot for the right tap at s == ns-1
-    double fixs = 1.0;
-    if constexpr (DET) fixs = *L.fix_scale;
+    double fixs[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) fixs[b] = 1.0;
+    if constexpr (DET) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (b < nb) fixs[b] = L.fix_scale[b0 + b];
+    }
     for (int e = tid; e < NB * rlen; e += BLOCK) rows[e] = Acc(0);
 
     const int ntiles = L.tiles_x * L.tiles_y;
@@ -541,10 +547,10 @@ __global__ __launch_bounds__(256) void bp_transpose_kernel(const float* __restri
                     if (b >= nb) break;
                     const float v = img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix];
                     if constexpr (DET) {
-                        atomicAdd(&rows[b * rlen + s], static_cast<unsigned long long>(
-                                      __double2ll_rn(static_cast<double>(w0 * v) * fixs)));
-                        atomicAdd(&rows[b * rlen + s + 1], static_cast<unsigned long long>(
-                                      __double2ll_rn(static_cast<double>(w1 * v) * fixs)));
+                        atomicAdd(&rows[b * rlen + s],
+                                  to_fixed(static_cast<double>(w0 * v) * fixs[b]));
+                        atomicAdd(&rows[b * rlen + s + 1],
+                                  to_fixed(static_cast<double>(w1 * v) * fixs[b]));
                     } else {
                         atomicAdd(&rows[b * rlen + s], w0 * v);
                         atomicAdd(&rows[b * rlen + s + 1], w1 * v);
@@ -557,10 +563,13 @@ __global__ __launch_bounds__(256) void bp_transpose_kernel(const float* __restri
     for (int e = tid; e < nb * P.ns; e += BLOCK) {
         const int b = e / P.ns, s = e - b * P.ns;
         double v;
-        if constexpr (DET)
-            v = static_cast<double>(static_cast<long long>(rows[b * rlen + s])) / fixs;
-        else
+        if constexpr (DET) {
+            const double f = L.fix_scale[b0 + b];
+            v = f != 0.0 ? static_cast<double>(static_cast<long long>(rows[b * rlen + s])) / f
+                         : __builtin_nan("");
+        } else {
             v = rows[b * rlen + s];
+        }
         proj[(static_cast<size_t>(b0 + b) * P.na + view) * P.ns + s] =
             static_cast<float>(v * L.scale);
     }

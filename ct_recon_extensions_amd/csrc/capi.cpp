@@ -7,9 +7,11 @@
 
 namespace ctr {
 int fan_bp_f32(const ctr_geom& g, const float* proj, float* img, int nbatch, const double* xpos,
-               const double* ypos, const double* sc, double scale, hipStream_t st);
+               const double* ypos, const double* sc, double scale, ctr_fan_bp_launch cfg,
+               hipStream_t st);
 int fan_bpt_f32(const ctr_geom& g, const float* img, float* proj, int nbatch, const double* xpos,
-                const double* ypos, const double* sc, double scale, bool det, hipStream_t st);
+                const double* ypos, const double* sc, double scale, bool det, void* ws,
+                hipStream_t st);
 int fan_bp_f64(const ctr_geom& g, const double* proj, double* img, int nbatch,
                const double* xpos, const double* ypos, const double* sc, double scale,
                hipStream_t st);
@@ -18,7 +20,7 @@ int fan_bpt_f64(const ctr_geom& g, const double* img, double* proj, int nbatch,
                 hipStream_t st);
 int plain_fan_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
                  const double* xpos, const double* ypos, const double* sc, double scale,
-                 hipStream_t st);
+                 int max_blocks, int max_threads, hipStream_t st);
 int par_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
            const double* xcos, const double* ysin, hipStream_t st);
 int par_bpt(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
@@ -32,7 +34,6 @@ int fp_par(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch
 int fp_par_t(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
              const double* w, const double* pos, const int* usex, unsigned int pos_stride,
              hipStream_t st);
-void fan_bp_set_config(int tile, int vchunk, int nb);
 int ramp(const ctr_geom& g, int dtype, const void* in, void* out, unsigned int nrow,
          const double* filter, int weight_type, hipStream_t st);
 int fan_weighting(const ctr_geom& g, int dtype, const void* in, void* out, unsigned int nrow,
@@ -77,32 +78,37 @@ int guarded(const char* who, F&& f)
 
 extern "C" {
 
-int ctr_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
-               const double* xpos, const double* ypos, const double* sincostbl, double scale,
-               void* stream)
+int ctr_fan_bp_ex(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                  const double* xpos, const double* ypos, const double* sincostbl, double scale,
+                  const ctr_fan_bp_launch* launch, void* stream)
 {
     int rc = check_common(g, dtype, proj, img, nbatch, "ctr_fan_bp");
     if (rc) return rc;
     CTR_REQUIRE(xpos && ypos && sincostbl, CTR_ERR_ARG, "ctr_fan_bp: null table");
+    const ctr_fan_bp_launch cfg = launch ? *launch : ctr_fan_bp_launch{0, 0, 0};
+    CTR_REQUIRE(cfg.tile >= 0 && cfg.tile <= 2, CTR_ERR_ARG,
+                "ctr_fan_bp_ex: launch tile must be 0, 1 or 2");
+    CTR_REQUIRE(cfg.vchunk >= 0 && cfg.vchunk <= 64, CTR_ERR_ARG,
+                "ctr_fan_bp_ex: launch vchunk must be in [0, 64]");
+    CTR_REQUIRE(cfg.nb == 0 || cfg.nb == 1 || cfg.nb == 2 || cfg.nb == 4 || cfg.nb == 8 ||
+                    cfg.nb == 16,
+                CTR_ERR_ARG, "ctr_fan_bp_ex: launch nb must be 0, 1, 2, 4, 8 or 16");
     return guarded("ctr_fan_bp", [&] {
         auto st = static_cast<hipStream_t>(stream);
         if (dtype == CTR_F32)
             return ctr::fan_bp_f32(*g, static_cast<const float*>(proj), static_cast<float*>(img),
-                                   nbatch, xpos, ypos, sincostbl, scale, st);
+                                   nbatch, xpos, ypos, sincostbl, scale, cfg, st);
         return ctr::fan_bp_f64(*g, static_cast<const double*>(proj), static_cast<double*>(img),
                                nbatch, xpos, ypos, sincostbl, scale, st);
     });
 }
 
-int ctr_fan_bp_config(int tile, int vchunk, int nb)
+int ctr_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+               const double* xpos, const double* ypos, const double* sincostbl, double scale,
+               void* stream)
 {
-    CTR_REQUIRE(tile >= 0 && tile <= 2, CTR_ERR_ARG, "ctr_fan_bp_config: tile must be 0, 1 or 2");
-    CTR_REQUIRE(vchunk >= 0 && vchunk <= 64, CTR_ERR_ARG,
-                "ctr_fan_bp_config: vchunk must be in [0, 64]");
-    CTR_REQUIRE(nb == 0 || nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16, CTR_ERR_ARG,
-                "ctr_fan_bp_config: nb must be 0, 1, 2, 4, 8 or 16");
-    ctr::fan_bp_set_config(tile, vchunk, nb);
-    return CTR_OK;
+    return ctr_fan_bp_ex(g, dtype, proj, img, nbatch, xpos, ypos, sincostbl, scale, nullptr,
+                         stream);
 }
 
 int ctr_fan_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
@@ -116,9 +122,27 @@ int ctr_fan_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int 
         auto st = static_cast<hipStream_t>(stream);
         if (dtype == CTR_F32)
             return ctr::fan_bpt_f32(*g, static_cast<const float*>(img), static_cast<float*>(proj),
-                                    nbatch, xpos, ypos, sincostbl, scale, deterministic != 0, st);
+                                    nbatch, xpos, ypos, sincostbl, scale, deterministic != 0,
+                                    nullptr, st);
         return ctr::fan_bpt_f64(*g, static_cast<const double*>(img), static_cast<double*>(proj),
                                 nbatch, xpos, ypos, sincostbl, scale, st);
+    });
+}
+
+int ctr_plain_fan_bp_grid(const ctr_geom* g, int dtype, const void* proj, void* img,
+                          int nbatch, const double* xpos, const double* ypos,
+                          const double* sincostbl, double scale, int max_blocks,
+                          int max_threads, void* stream)
+{
+    int rc = check_common(g, dtype, proj, img, nbatch, "ctr_plain_fan_bp");
+    if (rc) return rc;
+    CTR_REQUIRE(xpos && ypos && sincostbl, CTR_ERR_ARG, "ctr_plain_fan_bp: null table");
+    CTR_REQUIRE(max_blocks >= 0, CTR_ERR_ARG, "ctr_plain_fan_bp_grid: max_blocks must be >= 0");
+    CTR_REQUIRE(max_blocks == 0 || (max_threads >= 1 && max_threads <= 1024), CTR_ERR_ARG,
+                "ctr_plain_fan_bp_grid: max_threads must be in [1, 1024]");
+    return guarded("ctr_plain_fan_bp", [&] {
+        return ctr::plain_fan_bp(*g, dtype, proj, img, nbatch, xpos, ypos, sincostbl, scale,
+                                 max_blocks, max_threads, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -126,13 +150,8 @@ int ctr_plain_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, 
                      const double* xpos, const double* ypos, const double* sincostbl,
                      double scale, void* stream)
 {
-    int rc = check_common(g, dtype, proj, img, nbatch, "ctr_plain_fan_bp");
-    if (rc) return rc;
-    CTR_REQUIRE(xpos && ypos && sincostbl, CTR_ERR_ARG, "ctr_plain_fan_bp: null table");
-    return guarded("ctr_plain_fan_bp", [&] {
-        return ctr::plain_fan_bp(*g, dtype, proj, img, nbatch, xpos, ypos, sincostbl, scale,
-                                 static_cast<hipStream_t>(stream));
-    });
+    return ctr_plain_fan_bp_grid(g, dtype, proj, img, nbatch, xpos, ypos, sincostbl, scale, 0,
+                                 0, stream);
 }
 
 int ctr_par_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,

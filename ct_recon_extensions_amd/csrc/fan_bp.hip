@@ -21,6 +21,10 @@
 namespace ctr {
 
 // ------------------------------------------------- reference-structured path
+// One thread per pixel on a grid-stride loop (CudaGridRangeX,
+// cuda/cuda_common.h:83-91), FP64 tables read from global memory, FP64
+// atan2 and accumulation per view (cuda/bp_fan_2d.cu:72-110).  The launcher
+// chooses the grid: the reference's (at most 32 x 512 threads) or a full one.
 template <typename T>
 __global__ void plain_fan_bp_kernel(const T* __restrict__ proj, T* __restrict__ img,
                                     const double* __restrict__ xpos,
@@ -28,55 +32,113 @@ __global__ void plain_fan_bp_kernel(const T* __restrict__ proj, T* __restrict__ 
                                     const double* __restrict__ sincos, FanProblem P,
                                     double scale)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.nx * P.ny) return;
-    const int ix = i % P.nx, iy = i / P.nx;
-    const double x = xpos[ix], y = ypos[iy];
-    double sum = 0.0;
-    for (int ia = 0; ia < P.na; ++ia) {
-        const double sb = sincos[2 * ia], cb = sincos[2 * ia + 1];
-        const double d = P.dso + x * sb - y * cb;
-        const double r = x * cb + y * sb;
-        const double s = atan2(r, d) * P.dsd / P.ds + P.cents;
-        const double w = P.dsd * P.dsd / (d * d + r * r);
-        if (s >= 0 && s <= P.ns - 1) {
-            const double fl = floor(s);
-            const int is1 = static_cast<int>(fl), is2 = static_cast<int>(ceil(s));
-            const double u = s - fl;
-            sum += ((1 - u) * proj[is1 + ia * P.ns] + u * proj[is2 + ia * P.ns]) * w;
+    const int npix = P.nx * P.ny;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        const int ix = i % P.nx, iy = i / P.nx;
+        const double x = xpos[ix], y = ypos[iy];
+        double sum = 0.0;
+        for (int ia = 0; ia < P.na; ++ia) {
+            const double sb = sincos[2 * ia], cb = sincos[2 * ia + 1];
+            const double d = P.dso + x * sb - y * cb;
+            const double r = x * cb + y * sb;
+            const double s = atan2(r, d) * P.dsd / P.ds + P.cents;
+            const double w = P.dsd * P.dsd / (d * d + r * r);
+            if (s >= 0 && s <= P.ns - 1) {
+                const double fl = floor(s);
+                const int is1 = static_cast<int>(fl), is2 = static_cast<int>(ceil(s));
+                const double u = s - fl;
+                sum += ((1 - u) * proj[is1 + ia * P.ns] + u * proj[is2 + ia * P.ns]) * w;
+            }
         }
+        img[i] = static_cast<T>(sum * scale);
     }
-    img[i] = static_cast<T>(sum * scale);
 }
 
-// FP64 transpose for float64 buffers: one thread per pixel scattering with
-// FP64 global atomics (double is the reference's high-precision mode).
-__global__ void fan_bpt_f64_kernel(const double* __restrict__ img, double* __restrict__ proj,
-                                   const double* __restrict__ xpos,
-                                   const double* __restrict__ ypos,
-                                   const double* __restrict__ sincos, FanProblem P, int nbatch)
+// Interval of the (real) index t with p t + q >= 0, intersected into
+// [lo, hi]; p = 0 keeps everything or nothing by the sign of q.
+__device__ __forceinline__ void clip_halfline(double p, double q, double& lo, double& hi)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
-    if (i >= P.nx * P.ny || b >= nbatch) return;
-    const int ix = i % P.nx, iy = i / P.nx;
-    const double x = xpos[ix], y = ypos[iy];
-    const double v = img[static_cast<size_t>(b) * P.nx * P.ny + i];
-    double* out = proj + static_cast<size_t>(b) * P.na * P.ns;
-    for (int ia = 0; ia < P.na; ++ia) {
-        const double sb = sincos[2 * ia], cb = sincos[2 * ia + 1];
-        const double d = P.dso + x * sb - y * cb;
-        const double r = x * cb + y * sb;
-        const double w = P.dsd * P.dsd / (d * d + r * r);
-        const double s = atan2(r, d) * P.dsd / P.ds + P.cents;
-        if (s >= 0 && s <= P.ns - 1) {
-            const double fl = floor(s);
-            const double u = s - fl;
-            const int is1 = static_cast<int>(fl), is2 = static_cast<int>(ceil(s));
-            atomicAdd(&out[is1 + ia * P.ns], v * (w * (1 - u)));
-            atomicAdd(&out[is2 + ia * P.ns], v * (w * u));
+    if (p > 0.0)
+        lo = fmax(lo, -q / p);
+    else if (p < 0.0)
+        hi = fmin(hi, -q / p);
+    else if (q < 0.0)
+        hi = -1e300;
+}
+
+// FP64 transpose in gather form (float64 buffers, both modes): one thread
+// per (channel, view) and up to four slices sums, row by row and in
+// increasing ix, the contributions of the pixels whose position s falls in
+// [c-1, c+1].  Along an image row s is monotone (atan of a ratio of two
+// functions linear in x, with d > 0 over the image), so those pixels form an
+// interval, bounded here by the rays at c -+ (1 + 1/64) channels plus one
+// pixel of margin; every candidate is then evaluated with the forward
+// kernel's FP64 expressions.  The order of the sums is fixed: the result is
+// bitwise repeatable without atomics or a cleared output.
+constexpr int kGatherSlices = 4;
+
+__global__ void fan_bpt_f64_gather_kernel(const double* __restrict__ img,
+                                          double* __restrict__ proj,
+                                          const double* __restrict__ xpos,
+                                          const double* __restrict__ ypos,
+                                          const double* __restrict__ sincos, FanProblem P,
+                                          int nbatch, double scale)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ia = blockIdx.y, b0 = blockIdx.z * kGatherSlices;
+    if (c >= P.ns) return;
+    const int nb = min(kGatherSlices, nbatch - b0);
+    const double sb = sincos[2 * ia], cb = sincos[2 * ia + 1];
+    const double k = P.dsd / P.ds;
+    const double glo = (c - 1.015625 - P.cents) / k, ghi = (c + 1.015625 - P.cents) / k;
+    const double half_pi = 1.5707963267948966;
+    double acc[kGatherSlices] = {0.0, 0.0, 0.0, 0.0};
+    // |gamma| < pi/2 everywhere on the image: channels beyond are never hit
+    if (glo < half_pi && ghi > -half_pi) {
+        const bool has_lo = glo > -half_pi, has_hi = ghi < half_pi;
+        const double tlo = has_lo ? tan(glo) : 0.0, thi = has_hi ? tan(ghi) : 0.0;
+        // x(ix) ~ x0 + ix * xs (the table accumulates dx; the margin covers it)
+        const double x0 = xpos[0];
+        const double xs = P.nx > 1 ? (xpos[P.nx - 1] - x0) / (P.nx - 1) : 0.0;
+        for (int iy = 0; iy < P.ny; ++iy) {
+            const double y = ypos[iy];
+            double lo = -1.0, hi = P.nx;
+            // gamma >= glo  <=>  r - tan(glo) d >= 0, linear in x (d > 0)
+            if (has_lo) {
+                const double A = cb - tlo * sb, B = y * sb - tlo * (P.dso - y * cb);
+                clip_halfline(A * xs, A * x0 + B, lo, hi);
+            }
+            // gamma <= ghi  <=>  tan(ghi) d - r >= 0
+            if (has_hi) {
+                const double A = cb - thi * sb, B = y * sb - thi * (P.dso - y * cb);
+                clip_halfline(-A * xs, -(A * x0 + B), lo, hi);
+            }
+            if (!(lo <= hi)) continue;
+            const int ixl = max(0, static_cast<int>(floor(fmax(lo, -2.0))) - 1);
+            const int ixh = min(P.nx - 1, static_cast<int>(ceil(fmin(hi, P.nx + 2.0))) + 1);
+            for (int ix = ixl; ix <= ixh; ++ix) {
+                const double x = xpos[ix];
+                const double d = P.dso + x * sb - y * cb;
+                const double r = x * cb + y * sb;
+                const double s = atan2(r, d) * P.dsd / P.ds + P.cents;
+                if (!(s >= 0 && s <= P.ns - 1)) continue;
+                const double w = P.dsd * P.dsd / (d * d + r * r);
+                const double fl = floor(s);
+                const double u = s - fl;
+                const int is1 = static_cast<int>(fl), is2 = static_cast<int>(ceil(s));
+                const double wc = (is1 == c ? w * (1 - u) : 0.0) + (is2 == c ? w * u : 0.0);
+                if (wc == 0.0) continue;
+                const size_t pix = static_cast<size_t>(iy) * P.nx + ix;
+#pragma unroll
+                for (int b = 0; b < kGatherSlices; ++b)
+                    if (b < nb) acc[b] += img[static_cast<size_t>(b0 + b) * P.nx * P.ny + pix] * wc;
+            }
         }
     }
+#pragma unroll
+    for (int b = 0; b < kGatherSlices; ++b)
+        if (b < nb)
+            proj[(static_cast<size_t>(b0 + b) * P.na + ia) * P.ns + c] = acc[b] * scale;
 }
 
 // ---------------------------------------------------------------- host side
@@ -205,34 +267,20 @@ double fan_bp_scale(const ctr_geom& g)
     return std::fabs(g.orbit) / factor;
 }
 
-// Tunables of the forward kernel (set through ctr_fan_bp_config): tile
-// 0 = 16x16, 1 = 8x8, 2 = 32x8; views per LDS round (0 = engine_vchunk: on
-// the 16-slice 512^2 fan scan 6 views of 48-channel rows, 0.418 ms against
-// 0.453 for 4 views of 64); slices per workgroup (0 = automatic).
-struct FanBpConfig {
-    int tile = 0;
-    int vchunk = 0;
-    int nb = 0;
-};
-static FanBpConfig g_fan_cfg;
-
-void fan_bp_set_config(int tile, int vchunk, int nb)
-{
-    g_fan_cfg.tile = (tile >= 0 && tile <= 2) ? tile : 0;
-    g_fan_cfg.vchunk = (vchunk > 0 && vchunk <= kAnchorChunk) ? vchunk : 0;
-    g_fan_cfg.nb = (nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16) ? nb : 0;
-}
-
 static int auto_nb(int nbatch)
 {
     return nbatch >= 16 ? 16 : nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
 }
 
+// Launch shape of the forward kernel, per call: tile 0 = 16x16, 1 = 8x8,
+// 2 = 32x8; views per LDS round (0 = engine_vchunk: on the 16-slice 512^2
+// fan scan 6 views of 48-channel rows, 0.418 ms against 0.453 for 4 views of
+// 64); slices per workgroup (0 = from the batch).  The caller validated it.
 int fan_bp_f32(const ctr_geom& g, const float* proj, float* img, int nbatch, const double* xpos,
-               const double* ypos, const double* sc, double scale, hipStream_t st)
+               const double* ypos, const double* sc, double scale, ctr_fan_bp_launch cfg,
+               hipStream_t st)
 {
     static const int tw_of[3] = {16, 8, 32}, th_of[3] = {16, 8, 8};
-    FanBpConfig cfg = g_fan_cfg;
     FanProblem P{};
     int lmax = 0;
     bool exact = false;
@@ -290,8 +338,12 @@ static int launch_fan_bpt(const ctr_geom& g, const FanProblem& P, int nbatch, do
     return check_launch("bp_transpose_kernel<fan>");
 }
 
+// Deterministic mode needs fix_workspace_bytes(nbatch) bytes of device
+// workspace for the per-slice fixed-point scales: `ws` when the caller
+// brings it, else a stream-ordered allocation freed after the launch.
 int fan_bpt_f32(const ctr_geom& g, const float* img, float* proj, int nbatch, const double* xpos,
-                const double* ypos, const double* sc, double scale, bool det, hipStream_t st)
+                const double* ypos, const double* sc, double scale, bool det, void* ws,
+                hipStream_t st)
 {
     FanProblem P{};
     int lmax = 0;
@@ -300,23 +352,25 @@ int fan_bpt_f32(const ctr_geom& g, const float* img, float* proj, int nbatch, co
     if (rc) return rc;
     const FanTables T{xpos, ypos, sc};
     if (det) {
-        // Fixed-point scale from max|img| * max weight * contributions per channel.
-        float* partial = nullptr;
-        double* fs = nullptr;
-        CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), kMaxBlocks * sizeof(float), st));
-        CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&fs), sizeof(double), st));
-        const size_t n = static_cast<size_t>(nbatch) * g.nx * g.ny;
-        const int nblk = static_cast<int>(std::min<size_t>(kMaxBlocks, (n + 255) / 256));
-        hipLaunchKernelGGL(abs_max_partial_kernel, dim3(nblk), dim3(256), 0, st, img, n, partial);
+        // Per slice: 2^62 / (max|img| * max weight * 2 nx ny contributions per channel).
+        void* own = nullptr;
+        if (!ws) {
+            CTR_HIP(hipMallocAsync(&own, fix_workspace_bytes(nbatch), st));
+            ws = own;
+        }
         const double wmax = g.dsd / fan_min_distance(g, 16, 16);
         const double bound = 2.0 * g.nx * g.ny * std::max(1.0, wmax * wmax);
-        hipLaunchKernelGGL(fix_scale_kernel, dim3(1), dim3(64), 0, st, partial, nblk, bound, fs);
-        rc = check_launch("fix_scale_kernel");
+        hipError_t e = launch_fix_scales(img, nbatch, static_cast<size_t>(g.nx) * g.ny, bound, ws,
+                                         st);
+        if (e != hipSuccess) {
+            set_error(std::string("fix_scale_kernel: ") + hipGetErrorString(e));
+            rc = CTR_ERR_HIP;
+        }
+        const double* fs = static_cast<const double*>(ws);
         if (rc == CTR_OK)
             rc = exact ? launch_fan_bpt<true, 4, true>(g, P, nbatch, scale, fs, img, proj, T, st)
                        : launch_fan_bpt<false, 4, true>(g, P, nbatch, scale, fs, img, proj, T, st);
-        CTR_HIP(hipFreeAsync(partial, st));
-        CTR_HIP(hipFreeAsync(fs, st));
+        if (own) CTR_HIP(hipFreeAsync(own, st));
         return rc;
     }
     const int nb = nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
@@ -352,6 +406,7 @@ int fan_bp_f64(const ctr_geom& g, const double* proj, double* img, int nbatch,
     return CTR_OK;
 }
 
+// float64: the gather form, deterministic in both modes.
 int fan_bpt_f64(const ctr_geom& g, const double* img, double* proj, int nbatch,
                 const double* xpos, const double* ypos, const double* sc, double scale,
                 hipStream_t st)
@@ -361,20 +416,18 @@ int fan_bpt_f64(const ctr_geom& g, const double* img, double* proj, int nbatch,
     bool exact;
     int rc = fan_problem(g, 16, 16, P, lmax, exact);
     if (rc) return rc;
-    const size_t n = static_cast<size_t>(nbatch) * g.na * g.ns;
-    CTR_HIP(hipMemsetAsync(proj, 0, n * sizeof(double), st));
-    hipLaunchKernelGGL(fan_bpt_f64_kernel, dim3(div_up(g.nx * g.ny, 256), nbatch), dim3(256), 0,
-                       st, img, proj, xpos, ypos, sc, P, nbatch);
-    rc = check_launch("fan_bpt_f64_kernel");
-    if (rc) return rc;
-    hipLaunchKernelGGL(scale_f64_kernel, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256),
-                       0, st, proj, n, scale);
-    return check_launch("scale_f64_kernel");
+    hipLaunchKernelGGL(fan_bpt_f64_gather_kernel,
+                       dim3(div_up(g.ns, 64), g.na, div_up(nbatch, kGatherSlices)), dim3(64), 0,
+                       st, img, proj, xpos, ypos, sc, P, nbatch, scale);
+    return check_launch("fan_bpt_f64_gather_kernel");
 }
 
+// max_blocks = 0: a full grid of 256-thread workgroups; otherwise the
+// reference's GetCudaLaunchConfig (cuda/cuda_common.h:30-38) with the given
+// caps.  One launch per slice and a stream synchronisation after each.
 int plain_fan_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
                  const double* xpos, const double* ypos, const double* sc, double scale,
-                 hipStream_t st)
+                 int max_blocks, int max_threads, hipStream_t st)
 {
     FanProblem P{};
     int lmax;
@@ -382,15 +435,20 @@ int plain_fan_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int 
     int rc = fan_problem(g, 16, 16, P, lmax, exact);
     if (rc) return rc;
     const int npix = g.nx * g.ny;
+    int threads = 256, blocks = div_up(npix, 256);
+    if (max_blocks > 0) {
+        threads = std::min(max_threads, npix);
+        blocks = std::min(div_up(npix, threads), max_blocks);
+    }
     for (int b = 0; b < nbatch; ++b) {
         if (dtype == CTR_F32)
-            hipLaunchKernelGGL(plain_fan_bp_kernel<float>, dim3(div_up(npix, 256)), dim3(256), 0,
-                               st, static_cast<const float*>(proj) + static_cast<size_t>(b) * g.na * g.ns,
+            hipLaunchKernelGGL(plain_fan_bp_kernel<float>, dim3(blocks), dim3(threads), 0, st,
+                               static_cast<const float*>(proj) + static_cast<size_t>(b) * g.na * g.ns,
                                static_cast<float*>(img) + static_cast<size_t>(b) * npix, xpos, ypos,
                                sc, P, scale);
         else
-            hipLaunchKernelGGL(plain_fan_bp_kernel<double>, dim3(div_up(npix, 256)), dim3(256), 0,
-                               st, static_cast<const double*>(proj) + static_cast<size_t>(b) * g.na * g.ns,
+            hipLaunchKernelGGL(plain_fan_bp_kernel<double>, dim3(blocks), dim3(threads), 0, st,
+                               static_cast<const double*>(proj) + static_cast<size_t>(b) * g.na * g.ns,
                                static_cast<double*>(img) + static_cast<size_t>(b) * npix, xpos, ypos,
                                sc, P, scale);
         rc = check_launch("plain_fan_bp_kernel");

@@ -104,26 +104,64 @@ __global__ void par_bp_f64_kernel(const double* __restrict__ proj, double* __res
     img[static_cast<size_t>(b) * P.nx * P.ny + i] = sum * scale;
 }
 
-__global__ void par_bpt_f64_kernel(const double* __restrict__ img, double* __restrict__ proj,
-                                   const double* __restrict__ xcos,
-                                   const double* __restrict__ ysin, ParProblem P, int nbatch)
+// FP64 transpose in gather form (float64 buffers, both modes): one thread
+// per (channel, view) and up to four slices sums, row by row and in
+// increasing ix, the pixels whose position s falls in [c-1, c+1].  Along a
+// row s is affine in ix (xcos[ix][view] ~ x0 + ix * slope), so the
+// candidates form an interval, found with 1/64 channel and one pixel of
+// margin and then tested with the forward kernel's FP64 expressions.  Fixed
+// summation order: bitwise repeatable without atomics or a cleared output.
+constexpr int kParGatherSlices = 4;
+
+__global__ void par_bpt_f64_gather_kernel(const double* __restrict__ img,
+                                          double* __restrict__ proj,
+                                          const double* __restrict__ xcos,
+                                          const double* __restrict__ ysin, ParProblem P,
+                                          int nbatch, double scale)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
-    if (i >= P.nx * P.ny || b >= nbatch) return;
-    const int ix = i % P.nx, iy = i / P.nx;
-    const double v = img[static_cast<size_t>(b) * P.nx * P.ny + i];
-    double* out = proj + static_cast<size_t>(b) * P.na * P.ns;
-    for (int ia = 0; ia < P.na; ++ia) {
-        const double s = (xcos[static_cast<size_t>(ix) * P.na + ia] +
-                          ysin[static_cast<size_t>(iy) * P.na + ia]) / P.ds + P.cents;
-        if (s >= 0 && s <= P.ns - 1) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ia = blockIdx.y, b0 = blockIdx.z * kParGatherSlices;
+    if (c >= P.ns) return;
+    const int nb = min(kParGatherSlices, nbatch - b0);
+    const size_t na = P.na;
+    const double x0 = xcos[ia];
+    const double slope = P.nx > 1 ? (xcos[(P.nx - 1) * na + ia] - x0) / (P.nx - 1) : 0.0;
+    double acc[kParGatherSlices] = {0.0, 0.0, 0.0, 0.0};
+    for (int iy = 0; iy < P.ny; ++iy) {
+        const double ys = ysin[iy * na + ia];
+        // (c -+ (1 + 1/64) - cents) ds - ysin <= xcos <= ...
+        const double lo_v = (c - 1.015625 - P.cents) * P.ds - ys;
+        const double hi_v = (c + 1.015625 - P.cents) * P.ds - ys;
+        double lo = -1.0, hi = P.nx;
+        if (slope > 0.0) {
+            lo = fmax(lo, (lo_v - x0) / slope);
+            hi = fmin(hi, (hi_v - x0) / slope);
+        } else if (slope < 0.0) {
+            lo = fmax(lo, (hi_v - x0) / slope);
+            hi = fmin(hi, (lo_v - x0) / slope);
+        } else if (!(x0 >= lo_v && x0 <= hi_v)) {
+            continue;
+        }
+        if (!(lo <= hi)) continue;
+        const int ixl = max(0, static_cast<int>(floor(fmax(lo, -2.0))) - 1);
+        const int ixh = min(P.nx - 1, static_cast<int>(ceil(fmin(hi, P.nx + 2.0))) + 1);
+        for (int ix = ixl; ix <= ixh; ++ix) {
+            const double s = (xcos[ix * na + ia] + ys) / P.ds + P.cents;
+            if (!(s >= 0 && s <= P.ns - 1)) continue;
             const double fl = floor(s);
             const double u = s - fl;
-            atomicAdd(&out[static_cast<int>(fl) + ia * P.ns], (1 - u) * v);
-            atomicAdd(&out[static_cast<int>(ceil(s)) + ia * P.ns], u * v);
+            const int is1 = static_cast<int>(fl), is2 = static_cast<int>(ceil(s));
+            const double wc = (is1 == c ? 1 - u : 0.0) + (is2 == c ? u : 0.0);
+            if (wc == 0.0) continue;
+            const size_t pix = static_cast<size_t>(iy) * P.nx + ix;
+#pragma unroll
+            for (int b = 0; b < kParGatherSlices; ++b)
+                if (b < nb) acc[b] += wc * img[static_cast<size_t>(b0 + b) * P.nx * P.ny + pix];
         }
     }
+#pragma unroll
+    for (int b = 0; b < kParGatherSlices; ++b)
+        if (b < nb) proj[(static_cast<size_t>(b0 + b) * P.na + ia) * P.ns + c] = acc[b] * scale;
 }
 
 namespace {
@@ -240,36 +278,30 @@ int par_bpt(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatc
     L.tiles_y = div_up(g.ny, TH);
     L.scale = g.orbit;
     if (dtype == CTR_F64) {
-        const size_t n = static_cast<size_t>(nbatch) * g.na * g.ns;
-        CTR_HIP(hipMemsetAsync(proj, 0, n * sizeof(double), st));
-        hipLaunchKernelGGL(par_bpt_f64_kernel, dim3(div_up(g.nx * g.ny, 256), nbatch), dim3(256),
-                           0, st, static_cast<const double*>(img), static_cast<double*>(proj),
-                           xcos, ysin, L.P, nbatch);
-        rc = check_launch("par_bpt_f64_kernel");
-        if (rc) return rc;
-        hipLaunchKernelGGL(scale_f64_kernel, dim3(std::min<size_t>(4096, (n + 255) / 256)),
-                           dim3(256), 0, st, static_cast<double*>(proj), n, g.orbit);
-        return check_launch("scale_f64_kernel");
+        // the gather form: deterministic in both modes
+        hipLaunchKernelGGL(par_bpt_f64_gather_kernel,
+                           dim3(div_up(g.ns, 64), g.na, div_up(nbatch, kParGatherSlices)),
+                           dim3(64), 0, st, static_cast<const double*>(img),
+                           static_cast<double*>(proj), xcos, ysin, L.P, nbatch, g.orbit);
+        return check_launch("par_bpt_f64_gather_kernel");
     }
     const ParTables T{xcos, ysin};
     auto im = static_cast<const float*>(img);
     auto pr = static_cast<float*>(proj);
     const int ns = static_cast<int>(g.ns), na = static_cast<int>(g.na);
     if (det) {
-        float* partial = nullptr;
-        double* fs = nullptr;
-        CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), kMaxBlocks * sizeof(float), st));
-        CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&fs), sizeof(double), st));
-        const size_t n = static_cast<size_t>(nbatch) * g.nx * g.ny;
-        const int nblk = static_cast<int>(std::min<size_t>(kMaxBlocks, (n + 255) / 256));
-        hipLaunchKernelGGL(abs_max_partial_kernel, dim3(nblk), dim3(256), 0, st, im, n, partial);
-        hipLaunchKernelGGL(fix_scale_kernel, dim3(1), dim3(64), 0, st, partial, nblk,
-                           2.0 * g.nx * g.ny, fs);
-        L.fix_scale = fs;
-        rc = check_launch("fix_scale_kernel");
+        // per-slice fixed-point scales: 2^62 / (max|img| * 2 nx ny)
+        void* ws = nullptr;
+        CTR_HIP(hipMallocAsync(&ws, fix_workspace_bytes(nbatch), st));
+        hipError_t e = launch_fix_scales(im, nbatch, static_cast<size_t>(g.nx) * g.ny,
+                                         2.0 * g.nx * g.ny, ws, st);
+        if (e != hipSuccess) {
+            set_error(std::string("fix_scale_kernel: ") + hipGetErrorString(e));
+            rc = CTR_ERR_HIP;
+        }
+        L.fix_scale = static_cast<const double*>(ws);
         if (rc == CTR_OK) rc = launch_par_bpt<4, true>(L, ns, na, im, pr, T, st);
-        CTR_HIP(hipFreeAsync(partial, st));
-        CTR_HIP(hipFreeAsync(fs, st));
+        CTR_HIP(hipFreeAsync(ws, st));
         return rc;
     }
     const int nb = nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -148,8 +149,16 @@ struct ctr_ragged_plan {
     unsigned long long* d_stamps = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // Runs of a plan share its work counter.  Each run waits for this event
+    // (the end of the previous run) on its own stream and records it again,
+    // so runs from different streams or host threads execute one after the
+    // other instead of clearing the counter under each other; `mu` makes the
+    // wait / enqueue / record sequence and the first graph capture atomic.
+    hipEvent_t done = nullptr;
+    std::mutex mu;
     ~ctr_ragged_plan()
     {
+        if (done) (void)hipEventDestroy(done);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (d_descs) (void)hipFree(d_descs);
@@ -264,6 +273,7 @@ int ctr_ragged_fan_bp_plan(const ctr_ragged_problem* probs, int nprob, int tile,
         CTR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds));
         CTR_REQUIRE(per_cu >= 1, CTR_ERR_SHAPE, "ragged fan BP: kernel does not fit on a CU");
         p->grid = std::max(1, std::min(p->nitems, cus * per_cu));
+        CTR_HIP(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
         CTR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_descs), nprob * sizeof(RaggedDesc)));
         CTR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_items), items.size() * sizeof(RaggedItem)));
         CTR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_counter), 64));
@@ -286,7 +296,14 @@ int ctr_ragged_fan_bp_run(ctr_ragged_plan* plan, int use_graph, void* stream_)
     using namespace ctr;
     hipStream_t stream = static_cast<hipStream_t>(stream_);
     CTR_REQUIRE(plan, CTR_ERR_ARG, "ragged fan BP: null plan");
-    if (!use_graph) return enqueue(plan, stream, nullptr);
+    std::lock_guard<std::mutex> lock(plan->mu);
+    CTR_HIP(hipStreamWaitEvent(stream, plan->done, 0));
+    if (!use_graph) {
+        const int rc = enqueue(plan, stream, nullptr);
+        if (rc) return rc;
+        CTR_HIP(hipEventRecord(plan->done, stream));
+        return CTR_OK;
+    }
     if (!plan->exec) {
         // Capture on a private stream (the caller's may be the legacy null
         // stream, which cannot be captured); replay on the caller's.
@@ -319,6 +336,7 @@ int ctr_ragged_fan_bp_run(ctr_ragged_plan* plan, int use_graph, void* stream_)
         if (rc) return rc;
     }
     CTR_HIP(hipGraphLaunch(plan->exec, stream));
+    CTR_HIP(hipEventRecord(plan->done, stream));
     return CTR_OK;
 }
 
@@ -340,8 +358,11 @@ int ctr_ragged_fan_bp_balance(ctr_ragged_plan* plan, void* stream_, double* busy
     using namespace ctr;
     hipStream_t stream = static_cast<hipStream_t>(stream_);
     CTR_REQUIRE(plan && busy_fraction, CTR_ERR_ARG, "ragged fan BP: null argument");
+    std::lock_guard<std::mutex> lock(plan->mu);
+    CTR_HIP(hipStreamWaitEvent(stream, plan->done, 0));
     int rc = enqueue(plan, stream, plan->d_stamps);
     if (rc) return rc;
+    CTR_HIP(hipEventRecord(plan->done, stream));
     std::vector<unsigned long long> st(2 * static_cast<size_t>(plan->grid));
     CTR_HIP(hipMemcpyAsync(st.data(), plan->d_stamps, st.size() * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, stream));

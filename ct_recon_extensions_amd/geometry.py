@@ -206,6 +206,13 @@ def device_tables(kind: str, p, device, angles=None):
     hit = _CACHE.get(key)
     if hit is not None:
         _CACHE.move_to_end(key)
+        # Kernels about to read these tables may run on a stream other than
+        # the one that allocated them: tell the caching allocator, so that
+        # once the entry is evicted its memory is not handed out again
+        # before that stream's work is done.
+        st = torch.cuda.current_stream(hit[0].device)
+        for t in hit:
+            t.record_stream(st)
         return hit
     if kind == "fan":
         host = fan_tables(p)

@@ -108,20 +108,32 @@ int ctr_ramp_taps(const ctr_geom* g, double* filter);
 int ctr_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
                const double* xpos, const double* ypos, const double* sincostbl, double scale,
                void* stream);
-/* Launch configuration of ctr_fan_bp's float32 kernel for the calling
- * process: tile 0 = 16x16, 1 = 8x8, 2 = 32x8 pixels; vchunk = views per LDS
- * round (1..64, capped to the LDS budget; 0 = the most that keep four
+/* Launch shape of the float32 fan BP kernel, passed per call (no state is
+ * kept between calls, so concurrent calls with different shapes do not
+ * interact): tile 0 = 16x16, 1 = 8x8, 2 = 32x8 pixels; vchunk = views per
+ * LDS round (1..64, capped to the LDS budget; 0 = the most that keep four
  * workgroups per CU); nb = slices per workgroup (1, 2, 4, 8, 16; 0 = from
- * the batch).  Defaults 0, 0, 0.  Tiles whose
- * detector span exceeds 256 channels switch to 8x8 by themselves.  No
- * counterpart in the reference, whose launch shape is fixed
- * (cuda/bp_fan_2d.cu:219-245). */
-int ctr_fan_bp_config(int tile, int vchunk, int nb);
+ * the batch).  All zero is ctr_fan_bp's choice.  Tiles whose detector span
+ * exceeds 256 channels switch to 8x8 by themselves.  No counterpart in the
+ * reference, whose launch shape is fixed (cuda/bp_fan_2d.cu:219-245). */
+typedef struct ctr_fan_bp_launch {
+    int tile;
+    int vchunk;
+    int nb;
+} ctr_fan_bp_launch;
+/* ctr_fan_bp with an explicit launch shape (NULL = all zero).  The shape is
+ * validated before any device work; float64 buffers ignore it. */
+int ctr_fan_bp_ex(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                  const double* xpos, const double* ypos, const double* sincostbl, double scale,
+                  const ctr_fan_bp_launch* launch, void* stream);
 /* Exact transpose of ctr_fan_bp: img [nbatch][ny][nx] -> proj
  * [nbatch][na][ns] (replaces FanBackprojection2DPixDrivenGrad<T>,
  * src/bp_fan_2d.cc:124-170, and the broken bp_fan_2d_angle transpose).
- * View-parallel scatter; deterministic != 0 accumulates in 64-bit fixed
- * point so the result does not depend on the order of the contributions. */
+ * float32: view-parallel scatter; deterministic != 0 accumulates in 64-bit
+ * fixed point with one scale per slice, so the result does not depend on
+ * the order of the contributions (a slice holding Inf or NaN then yields
+ * NaN throughout).  float64: a gather over the pixels that reach each
+ * channel, in a fixed order, so it is bitwise repeatable in either mode. */
 int ctr_fan_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
                  const double* xpos, const double* ypos, const double* sincostbl, double scale,
                  int deterministic, void* stream);
@@ -140,7 +152,8 @@ int ctr_par_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nb
 int ctr_par_sv_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
                   const double* xcos, const double* ysin, void* stream);
 /* Exact transpose of ctr_par_bp (replaces the reference's broken
- * ParallelBackprojection2DPixDrivenGrad<T>, src/bp_par_2d.cc:147-199). */
+ * ParallelBackprojection2DPixDrivenGrad<T>, src/bp_par_2d.cc:147-199);
+ * deterministic and float64 as for ctr_fan_bp_t. */
 int ctr_par_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
                  const double* xcos, const double* ysin, int deterministic, void* stream);
 /* Ray-driven parallel FP, ParallelProjection2DRayDriven<T>
@@ -191,6 +204,15 @@ int ctr_fan_weighting(const ctr_geom* g, int dtype, const void* in, void* out,
 int ctr_plain_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
                      const double* xpos, const double* ypos, const double* sincostbl,
                      double scale, void* stream);
+/* The same with the reference's launch geometry (GetCudaLaunchConfig and
+ * CudaGridRangeX, cuda/cuda_common.h:30-91): threads = min(max_threads,
+ * npix), blocks = min(ceil(npix / threads), max_blocks), each thread walking
+ * pixels with a grid stride.  The reference uses max_blocks = 32 and
+ * max_threads = 512; max_blocks = 0 means a full grid (ctr_plain_fan_bp). */
+int ctr_plain_fan_bp_grid(const ctr_geom* g, int dtype, const void* proj, void* img,
+                          int nbatch, const double* xpos, const double* ypos,
+                          const double* sincostbl, double scale, int max_blocks,
+                          int max_threads, void* stream);
 
 /* ---------------------------------------------------------------------
  * Ragged batches of FanBackprojection2DAnglePixDriven<float>
@@ -203,7 +225,9 @@ int ctr_plain_fan_bp(const ctr_geom* g, int dtype, const void* proj, void* img, 
  * work counter with a kernel and launches the persistent kernel, either
  * directly or (use_graph != 0) by replaying a HIP graph captured on the
  * first such call.  img = scale * sum over views, scale = 1 for the
- * reference's bp_fan_2d_angle.  Runs of one plan must not overlap.
+ * reference's bp_fan_2d_angle.  Runs of one plan share its work counter,
+ * so the library orders them: each run waits (on its stream) for the end of
+ * the plan's previous run, whichever stream or host thread issued it. 
  * ------------------------------------------------------------------- */
 typedef struct ctr_ragged_problem {
     ctr_geom g;                 /* ns, na, ds, offset_s, nx, ny, dx, dy, offsets, dso, dsd */

@@ -128,13 +128,40 @@ def test_error_paths_return_codes(lib):
     assert rc == 1 and "weight_type" in _lib.last_error()
 
 
-def test_fan_bp_config_validates_arguments(lib):
+def test_fan_bp_launch_shape_is_validated_per_call_on_each_thread(lib):
+    """The fan BP's launch shape travels with each call (ctr_fan_bp_ex): no
+    process-wide setter exists, and two host threads passing different bad
+    shapes at once each get their own status and message (thread-local),
+    before any device work."""
+    import threading
+
     from ct_recon_extensions_amd import _lib
-    for bad, word in (((3, 0, 0), "tile"), ((0, -1, 0), "vchunk"), ((0, 65, 0), "vchunk"),
-                      ((0, 0, 3), "nb")):
-        assert lib.ctr_fan_bp_config(*bad) == 1 and word in _lib.last_error(), bad
-    assert lib.ctr_fan_bp_config(2, 6, 8) == 0
-    assert lib.ctr_fan_bp_config(0, 0, 0) == 0  # back to the defaults
+    assert not hasattr(lib, "ctr_fan_bp_config")
+    g = _geom(ns=8, na=4, nx=4, ny=4, ds=1.0, dx=1.0, dy=1.0, dso=50.0, dsd=90.0)
+    cases = {"tile": _lib.CtrFanBpLaunch(3, 0, 0), "vchunk": _lib.CtrFanBpLaunch(0, 65, 0),
+             "nb": _lib.CtrFanBpLaunch(0, 0, 3)}
+    errors = []
+
+    def worker(word, launch):
+        for _ in range(300):
+            rc = lib.ctr_fan_bp_ex(C.byref(g), 0, C.c_void_p(1), C.c_void_p(1), 1,
+                                   C.c_void_p(1), C.c_void_p(1), C.c_void_p(1), 1.0,
+                                   C.byref(launch), None)
+            msg = _lib.last_error()
+            if rc != 1 or word not in msg:
+                errors.append((word, rc, msg))
+                return
+
+    threads = [threading.Thread(target=worker, args=kv) for kv in cases.items()]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    # plain path: the reference grid's caps are checked the same way
+    rc = lib.ctr_plain_fan_bp_grid(C.byref(g), 0, C.c_void_p(1), C.c_void_p(1), 1, C.c_void_p(1),
+                                   C.c_void_p(1), C.c_void_p(1), 1.0, 32, 0, None)
+    assert rc == 1 and "max_threads" in _lib.last_error()
 
 
 def test_python_ops_refuse_cpu_tensors(lib):

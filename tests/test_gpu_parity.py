@@ -100,22 +100,32 @@ def test_fan_bp_batches_match_per_slice_oracle(F, nbatch):
 # The launch shape (tile, views per LDS round, slices per workgroup) does not
 # change the result beyond float32 rounding: every configuration against the
 # oracle, including view chunks past the LDS budget (capped) and 48- and
-# 64-channel span rows.
+# 64-channel span rows.  The shape travels with the call (ctr_fan_bp_ex).
 @pytest.mark.parametrize("tile,vchunk,nb", [(0, 0, 0), (0, 1, 4), (0, 6, 16), (0, 64, 8),
                                             (1, 0, 0), (2, 5, 16), (1, 3, 2)])
 def test_fan_bp_launch_configs_match_oracle(F, tile, vchunk, nb):
+    import ctypes as C
     from ct_recon_extensions_amd import _lib
+    from ct_recon_extensions_amd.functional import _fan_param
+    from ct_recon_extensions_amd.geometry import device_tables
     from tests.golden.make_golden import FAN_SMALL
     geo = dict(FAN_SMALL, fov=-1.0)
     g = O.geom(**FAN_SMALL)
     t = O.fan_prep(g)
     proj = randn(700 + vchunk, (16, g.na, g.ns))
-    lib = _lib.load()
-    try:
-        assert lib.ctr_fan_bp_config(tile, vchunk, nb) == 0
-        out = host(F.fan_backprojection_2d(cuda(proj), **fan_kwargs(geo)))
-    finally:
-        lib.ctr_fan_bp_config(0, 0, 0)
+    p = _fan_param([geo["ny"], geo["nx"]], [geo["dy"], geo["dx"]], [geo["na"], geo["ns"]],
+                   geo["ds"], [geo["offset_y"], geo["offset_x"]], geo["offset_s"],
+                   geo["orbit_start"], geo["orbit"], geo["dso"], geo["dsd"], -1)
+    x = cuda(proj)
+    out = torch.empty((16, g.ny, g.nx), device="cuda")
+    xpos, ypos, sc = device_tables("fan", p, x.device)
+    launch = _lib.CtrFanBpLaunch(tile, vchunk, nb)
+    _lib.check(_lib.load().ctr_fan_bp_ex(
+        C.byref(p.ctr()), 0, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()), 16,
+        C.c_void_p(xpos.data_ptr()), C.c_void_p(ypos.data_ptr()), C.c_void_p(sc.data_ptr()),
+        p.scale, C.byref(launch), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+        "ctr_fan_bp_ex")
+    out = host(out)
     for b in range(16):
         assert rel_l2(out[b], O.fan_bp(g, proj[b], t)) < TOL32, b
 
