@@ -66,6 +66,7 @@ _SIGS = {
     "ctr_fp_tables": (C.c_int, [_G, _P, _P, _P]),
     "ctr_fp_rc_tables": (C.c_int, [_G, C.c_double, _P, _P, _P]),
     "ctr_fp_grad_tables": (C.c_int, [_G, _P, _P, _P, C.c_uint]),
+    "ctr_fp_dd_tables": (C.c_int, [_G, _P, _P, _P]),
     "ctr_ramp_taps": (C.c_int, [_G, _P]),
     "ctr_fan_bp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double, _P]),
     "ctr_fan_bp_ex": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_double,
@@ -80,6 +81,8 @@ _SIGS = {
     "ctr_fp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
     "ctr_fp_rc": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, C.c_double, _P, _P, _P, _P]),
     "ctr_fp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_uint, _P]),
+    "ctr_fp_dd": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
+    "ctr_fp_dd_grad": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_uint, _P]),
     "ctr_ramp": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, _P]),
     "ctr_fan_weighting": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P]),
     "ctr_ramp_toeplitz": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, C.c_int, _P]),

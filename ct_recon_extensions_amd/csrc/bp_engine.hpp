@@ -473,106 +473,4 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) void bp_forward_kernel(
     bp_tile<Geo, TW, TH, NB, LROW>(T, L, tile, group * NB, smem);
 }
 
-// View-parallel scatter transpose.  One workgroup owns the detector rows of
-// one view for NB slices in LDS, walks every tile with the forward kernel's
-// tiling and anchors (bit-identical weights), scatters with LDS atomics and
-// writes the rows once.  DET: 64-bit fixed-point accumulation (integer
-// atomics are associative, so the result is independent of ordering).
-template <class Geo>
-struct BptLaunch {
-    typename Geo::Problem P;
-    int nbatch, tiles_x, tiles_y;
-    double scale;
-    const double* fix_scale;  // DET: device array of the per-slice scales 2^E
-};
-
-template <class Geo, int TW, int TH, int NB, bool DET>
-__global__ __launch_bounds__(256) void bp_transpose_kernel(const float* __restrict__ img,
-                                                           float* __restrict__ proj,
-                                                           const typename Geo::Tables T,
-                                                           const BptLaunch<Geo> L)
-{
-    using TS = TileShape<TW, TH>;
-    using Anchor = typename Geo::Anchor;
-    constexpr int BLOCK = TS::kBlock, PPT = TS::kPPT;
-    static_assert(BLOCK == 256, "transpose assumes 256-thread tiles");
-    using Acc = typename std::conditional<DET, unsigned long long, float>::type;
-    extern __shared__ __align__(16) unsigned char smem[];
-    Anchor* anchors = reinterpret_cast<Anchor*>(smem);
-    Acc* rows = reinterpret_cast<Acc*>(smem + kAnchorChunk * sizeof(Anchor));
-    const auto& P = L.P;
-    const int tid = threadIdx.x;
-    const int view = blockIdx.x;
-    const int b0 = blockIdx.y * NB;
-    const int nb = min(NB, L.nbatch - b0);
-    const int rlen = P.ns + 1;  // pad slot for the right tap at s == ns-1
-    double fixs[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) fixs[b] = 1.0;
-    if constexpr (DET) {
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-            if (b < nb) fixs[b] = L.fix_scale[b0 + b];
-    }
-    for (int e = tid; e < NB * rlen; e += BLOCK) rows[e] = Acc(0);
-
-    const int ntiles = L.tiles_x * L.tiles_y;
-    for (int tc = 0; tc < ntiles; tc += kAnchorChunk) {
-        const int ntc = min(kAnchorChunk, ntiles - tc);
-        __syncthreads();
-        if (tid < ntc) {
-            const TileFrame F(tc + tid, L.tiles_x, P.nx, P.ny, TW, TH);
-            anchors[tid] = Geo::anchor(P, T, F, view, F.x0 - F.ic, F.x0 + TW - 1 - F.ic,
-                                       F.y0 - F.jc, F.y0 + TH - 1 - F.jc);
-        }
-        __syncthreads();
-        for (int t = 0; t < ntc; ++t) {
-            const Anchor A = anchors[t];
-            const TileFrame F(tc + t, L.tiles_x, P.nx, P.ny, TW, TH);
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                const int p = tid + k * BLOCK;
-                const int ix = F.x0 + p % TW, iy = F.y0 + p / TW;
-                if (ix >= P.nx || iy >= P.ny) continue;
-                float f, w;
-                Geo::pixel(P, A, static_cast<float>(ix - F.ic), static_cast<float>(iy - F.jc), f,
-                           w);
-                const float fl = floorf(f);
-                const float fr = f - fl;
-                const int s = A.is0 + static_cast<int>(fl);
-                if (!on_detector(s, fr, P.ns)) continue;
-                const float w1 = w * fr, w0 = w - w1;
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    if (b >= nb) break;
-                    const float v = img[(static_cast<size_t>(b0 + b) * P.ny + iy) * P.nx + ix];
-                    if constexpr (DET) {
-                        atomicAdd(&rows[b * rlen + s],
-                                  to_fixed(static_cast<double>(w0 * v) * fixs[b]));
-                        atomicAdd(&rows[b * rlen + s + 1],
-                                  to_fixed(static_cast<double>(w1 * v) * fixs[b]));
-                    } else {
-                        atomicAdd(&rows[b * rlen + s], w0 * v);
-                        atomicAdd(&rows[b * rlen + s + 1], w1 * v);
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < nb * P.ns; e += BLOCK) {
-        const int b = e / P.ns, s = e - b * P.ns;
-        double v;
-        if constexpr (DET) {
-            const double f = L.fix_scale[b0 + b];
-            v = f != 0.0 ? static_cast<double>(static_cast<long long>(rows[b * rlen + s])) / f
-                         : __builtin_nan("");
-        } else {
-            v = rows[b * rlen + s];
-        }
-        proj[(static_cast<size_t>(b0 + b) * P.na + view) * P.ns + s] =
-            static_cast<float>(v * L.scale);
-    }
-}
-
 }  // namespace ctr

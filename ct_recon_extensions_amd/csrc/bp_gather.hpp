@@ -1,0 +1,481 @@
+// Gather-form exact transpose of the pixel-driven BP engine, for float32.
+//
+// The BP sums, for every pixel and view, w * lerp(proj[view], s) with
+// s = s(pixel, view) and the taps floor(s), floor(s)+1.  Its transpose
+// therefore gives channel c of a view the sum of img * w * (1 - frac) over
+// the pixels with floor(s) = c, plus img * w * frac over those with
+// floor(s) = c - 1: the pixels whose s lies in (c-1, c+1).  Here one lane
+// owns one (view, channel) output and adds those contributions in registers,
+// in a fixed order (image rows in increasing order, pixels along a row in
+// increasing order): no atomics, no cleared output, and the result is
+// bitwise repeatable.
+//
+// Finding the pixels: the level line s = s* of both geometries is a
+// straight line in pixel-index space (a ray: r - tan(gamma*) d = 0 for the
+// fan, x cos + y sin = const for parallel beams), written L(ix, iy) =
+// pi ix + qj iy + rr with L >= 0 exactly where s >= s* (Geo::boundary).  On
+// one image row the pixels with s in [c-1-eps, c+1+eps] are then the
+// intersection of two half-lines, computed per row from two linear bounds;
+// each candidate is evaluated with the forward kernel's own per-tile
+// anchors and FP32 pixel geometry (Geo::pixel), so the weights are those of
+// the forward.  The margin eps (1/32 channel) keeps every pixel of the
+// support inside the candidate interval despite FP32 rounding of the bounds.
+//
+// Work layout (gfx950): a workgroup = 4 views (one wave each) x 64 channels
+// (one lane each) x NB slices.  It walks the image in bands of R rows; per
+// band the union of its lanes' candidate intervals is staged into LDS in
+// chunks of WMAX pixels (slice-interleaved: [row][slice group][pixel][4
+// slices], one ds_read_b128 per 4 slices), together with the anchors of the
+// 16x16 tiles the chunk covers (from a table built once per call).  Views
+// whose rays run closer to the image rows than to the columns walk the
+// columns instead ("frame 1"), on a transposed copy of the image, so that a
+// row's candidates are always a short interval.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bp_engine.hpp"
+#include "ctr_common.hpp"
+#include "tile_common.hpp"
+
+namespace ctr {
+
+constexpr int kGatherTile = 16;     // anchors of 16 x 16 tiles (the forward's default)
+constexpr int kGatherViews = 4;     // views per workgroup (one wave each)
+constexpr int kGatherLanes = 64;    // channels per workgroup (one lane each)
+constexpr double kGatherEps = 1.0 / 32;
+
+// anchors[view][tile] of 16 x 16 tiles: the FP64 per-(tile, view) geometry,
+// evaluated once per call.
+template <class Geo>
+__global__ void bp_anchor_table_kernel(const typename Geo::Problem P, const typename Geo::Tables T,
+                                       int tiles_x, int ntiles, typename Geo::Anchor* __restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (t >= ntiles) return;
+    const TileFrame F(t, tiles_x, P.nx, P.ny, kGatherTile, kGatherTile);
+    out[static_cast<size_t>(v) * ntiles + t] =
+        Geo::anchor(P, T, F, v, F.x0 - F.ic, F.x0 + kGatherTile - 1 - F.ic, F.y0 - F.jc,
+                    F.y0 + kGatherTile - 1 - F.jc);
+}
+
+// dst[b][x][y] = src[b][y][x] (the image seen column by column).
+static __global__ void image_transpose_kernel(const float* __restrict__ src,
+                                              float* __restrict__ dst, int ny, int nx)
+{
+    __shared__ float tile[32][33];
+    const size_t base = static_cast<size_t>(blockIdx.z) * nx * ny;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int y = y0 + r, x = x0 + threadIdx.x;
+        if (y < ny && x < nx) tile[r][threadIdx.x] = src[base + static_cast<size_t>(y) * nx + x];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int x = x0 + r, y = y0 + threadIdx.x;
+        if (y < ny && x < nx) dst[base + static_cast<size_t>(x) * ny + y] = tile[threadIdx.x][r];
+    }
+}
+
+template <class Geo>
+struct GatherLaunch {
+    typename Geo::Problem P;
+    typename Geo::Tables T;
+    const typename Geo::Anchor* anchors;  // [na][tiles_x * tiles_y]
+    const float* img;                     // [nbatch][ny][nx]
+    const float* imgT;                    // [nbatch][nx][ny]
+    float* proj;                          // [nbatch][na][ns]
+    int nbatch, tiles_x, tiles_y;
+    double scale;
+};
+
+// One linear constraint of a lane's candidates along a row: w >= A + B u
+// (lower) or w <= A + B u (upper); none = no constraint from this line.
+struct GatherBound {
+    double A, B;
+    bool lower, none;
+};
+
+// Constraint sign * L >= 0 in frame coordinates (u walked, w along the row):
+// L = cw w + cu u + rr.
+__device__ inline GatherBound gather_bound(double cw, double cu, double rr, double sign)
+{
+    cw *= sign;
+    cu *= sign;
+    rr *= sign;
+    GatherBound b;
+    const double mag = fabs(cw) + fabs(cu);
+    if (!(fabs(cw) > 1e-12 * mag) || !(mag > 0)) {
+        // parallel to the rows (or a degenerate one-pixel axis): no bound
+        b.A = b.B = 0.0;
+        b.lower = true;
+        b.none = true;
+        return b;
+    }
+    b.A = -rr / cw;
+    b.B = -cu / cw;
+    b.lower = cw > 0;
+    b.none = false;
+    return b;
+}
+
+template <class Geo, int NB, int R, int WMAX>
+struct GatherShape {
+    static constexpr bool kQuad = NB % 4 == 0;
+    static constexpr int kGroups = kQuad ? NB / 4 : 1;
+    static constexpr int kTiles = WMAX / kGatherTile;  // tiles along w per chunk
+    static constexpr int kImgFloats = R * WMAX * NB;
+    static constexpr int kAnchorDwords =
+        kGatherViews * kTiles * static_cast<int>(sizeof(typename Geo::Anchor) / 4);
+    static constexpr size_t kLds = kImgFloats * sizeof(float) + kAnchorDwords * 4 + 64;
+    static_assert(NB == 1 || kQuad, "slices per workgroup: 1 or a multiple of 4");
+    static_assert(WMAX % (kQuad ? 16 : 64) == 0, "chunk width: whole DMA rows");
+    static_assert(kGatherTile % R == 0, "bands must not straddle a tile row");
+};
+
+// LDS (about 34 KB at 16 slices) keeps four workgroups on a CU, i.e. four
+// waves per SIMD: the registers of that occupancy (128) are available.
+template <class Geo, int NB, int R, int WMAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void bp_gather_t_kernel(const GatherLaunch<Geo> L)
+{
+    using S = GatherShape<Geo, NB, R, WMAX>;
+    using Anchor = typename Geo::Anchor;
+    constexpr int V = kGatherViews, K = S::kTiles, NG = S::kGroups;
+    constexpr int DW = static_cast<int>(sizeof(Anchor) / 4);
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* simg = reinterpret_cast<float*>(smem);
+    unsigned int* sanc = reinterpret_cast<unsigned int*>(smem + S::kImgFloats * sizeof(float));
+    int* sred = reinterpret_cast<int*>(smem + S::kImgFloats * sizeof(float) + S::kAnchorDwords * 4);
+    const Anchor* anch = reinterpret_cast<const Anchor*>(sanc);
+
+    const auto& P = L.P;
+    typename Geo::Problem Pg = P;
+    if constexpr (HasPin<Geo>::value) Geo::pin(Pg);
+    const int tid = threadIdx.x, lane = tid & 63, wave = uniform_i(tid >> 6);
+    const int c0 = blockIdx.x * kGatherLanes, v0 = blockIdx.y * V, b0 = blockIdx.z * NB;
+    const int c = c0 + lane, view = v0 + wave;
+    const int nb = min(NB, L.nbatch - b0);
+    const bool active = c < P.ns && view < P.na;
+
+    // Frame of the workgroup, from the level line through its middle view
+    // and channel: walk rows (0) when that line is closer to the columns.
+    int frame;
+    {
+        const int vm = v0 + (min(V, P.na - v0) - 1) / 2;
+        const int cm = c0 + (min(kGatherLanes, P.ns - c0) - 1) / 2;
+        double pi, qj, rr;
+        Geo::boundary(P, L.T, vm, static_cast<double>(cm), pi, qj, rr);
+        frame = uniform_i(fabs(pi) >= fabs(qj) ? 0 : 1);
+    }
+    const int nu = frame ? P.nx : P.ny, nw = frame ? P.ny : P.nx;
+    const float* src = frame ? L.imgT : L.img;
+    const unsigned int slice_bytes = static_cast<unsigned int>(nu) * nw * 4u;
+
+    // This lane's two constraints: s >= c - 1 - eps and s <= c + 1 + eps.
+    GatherBound bd[2];
+    {
+        const int vv = active ? view : min(view, P.na - 1);
+        const double sl[2] = {c - 1 - kGatherEps, c + 1 + kGatherEps};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            double pi, qj, rr;
+            Geo::boundary(P, L.T, vv, sl[k], pi, qj, rr);
+            const double cw = frame ? qj : pi, cu = frame ? pi : qj;
+            bd[k] = gather_bound(cw, cu, rr, k == 0 ? 1.0 : -1.0);
+        }
+    }
+
+    float acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = 0.0f;
+
+    const int nbands = (nu + R - 1) / R;
+    for (int band = 0; band < nbands; ++band) {
+        const int u0 = band * R;
+        const int rmax = min(R, nu - u0);
+        // Bounds re-anchored at the band's first row (FP64), then FP32 per row.
+        float fa[2], fb[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            fa[k] = static_cast<float>(bd[k].A + bd[k].B * u0);
+            fb[k] = static_cast<float>(bd[k].B);
+        }
+        // This lane's candidate range over the band's rows (a superset):
+        // lower bounds are linear in the row, so their smallest value is at
+        // an end row; likewise the largest upper bound.
+        float lo = 3.0e38f, hi = -3.0e38f;
+        if (active) {
+            float lmin = 3.0e38f, umax = -3.0e38f;
+            bool has_l = false, has_u = false;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (bd[k].none) continue;
+                const float e0 = fa[k], e1 = fmaf(fb[k], static_cast<float>(rmax - 1), fa[k]);
+                if (bd[k].lower) {
+                    lmin = fminf(lmin, fminf(e0, e1));
+                    has_l = true;
+                } else {
+                    umax = fmaxf(umax, fmaxf(e0, e1));
+                    has_u = true;
+                }
+            }
+            lo = has_l ? lmin : 0.0f;
+            hi = has_u ? umax : static_cast<float>(nw - 1);
+        }
+        int ilo = static_cast<int>(floorf(fmaxf(lo, -1.0f)));
+        int ihi = static_cast<int>(ceilf(fminf(hi, static_cast<float>(nw))));
+        ilo = max(ilo, 0);
+        ihi = min(ihi, nw - 1);
+        if (ilo > ihi) {
+            ilo = 1 << 30;
+            ihi = -(1 << 30);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            ilo = min(ilo, __shfl_xor(ilo, o));
+            ihi = max(ihi, __shfl_xor(ihi, o));
+        }
+        // previous band's candidates are done with the LDS before this
+        // barrier; the reduction slots alternate between bands
+        int* red = sred + (band & 1) * 2 * V;
+        if (lane == 0) {
+            red[2 * wave] = ilo;
+            red[2 * wave + 1] = ihi;
+        }
+        __syncthreads();
+        int W0 = red[0], W1 = red[1];
+#pragma unroll
+        for (int w = 1; w < V; ++w) {
+            W0 = min(W0, red[2 * w]);
+            W1 = max(W1, red[2 * w + 1]);
+        }
+        W0 = uniform_i(W0);
+        W1 = uniform_i(W1);
+        if (W0 > W1) continue;
+
+        for (int cw = W0 & ~(kGatherTile - 1); cw <= W1; cw += WMAX) {
+            if (cw != (W0 & ~(kGatherTile - 1))) __syncthreads();  // previous chunk done
+            // ---- stage the image: rows u0 .. u0+rmax-1, pixels cw .. cw+WMAX-1
+            if constexpr (S::kQuad) {
+                constexpr int JOBS = R * NG * K;
+                const int lane_px = lane >> 2;
+                const unsigned int lane_off = (lane & 3) * slice_bytes;
+                for (int j = wave; j < JOBS; j += V) {
+                    const int r = j / (NG * K), g = (j / K) % NG, k = j % K;
+                    const int ng = nb - 4 * g;
+                    if (r >= rmax || ng <= 0) continue;
+                    const float* base = src + (static_cast<size_t>(b0 + 4 * g) * nu + u0 + r) * nw;
+                    const unsigned int range = ng >= 4 ? 4u * slice_bytes : ng * slice_bytes;
+                    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                        const_cast<float*>(base), 0, range, 0x00020000);
+                    const int px = cw + 16 * k + lane_px;
+                    const unsigned int voff = static_cast<unsigned int>(px) < static_cast<unsigned int>(nw)
+                                                  ? lane_off + static_cast<unsigned int>(px) * 4u
+                                                  : 0x80000000u;
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rs,
+                        (__attribute__((address_space(3))) void*)(simg +
+                                                                  ((r * NG + g) * WMAX + 16 * k) * 4),
+                        4, voff, 0, 0, 0);
+                }
+            } else {
+                constexpr int JOBS = R * (WMAX / 64);
+                for (int j = wave; j < JOBS; j += V) {
+                    const int r = j / (WMAX / 64), k = j % (WMAX / 64);
+                    if (r >= rmax || nb <= 0) continue;
+                    const float* base = src + (static_cast<size_t>(b0) * nu + u0 + r) * nw;
+                    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                        const_cast<float*>(base), 0, static_cast<unsigned int>(nw) * 4u, 0x00020000);
+                    const int px = cw + 64 * k + lane;
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rs, (__attribute__((address_space(3))) void*)(simg + r * WMAX + 64 * k), 4,
+                        static_cast<unsigned int>(px) * 4u, 0, 0, 0);
+                }
+            }
+            // ---- the anchors of the chunk's tiles, for the workgroup's views
+            {
+                const int ntiles = L.tiles_x * L.tiles_y;
+                const int tu = u0 / kGatherTile, tw0 = cw / kGatherTile;
+                const int tw_max = frame ? L.tiles_y : L.tiles_x;
+                const unsigned int* tab = reinterpret_cast<const unsigned int*>(L.anchors);
+                for (int d = tid; d < S::kAnchorDwords; d += 256) {
+                    const int a = d / DW, word = d - a * DW;
+                    const int v = a / K, k = a - v * K;
+                    const int tw = tw0 + k;
+                    unsigned int val = 0u;
+                    if (v0 + v < P.na && tw < tw_max) {
+                        const int tile = frame ? tw * L.tiles_x + tu : tu * L.tiles_x + tw;
+                        val = tab[(static_cast<size_t>(v0 + v) * ntiles + tile) * DW + word];
+                    }
+                    sanc[d] = val;
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+
+            // ---- this lane's candidates: the row intervals first, then one
+            // counted loop over all of them (candidate k -> row r by the
+            // prefix counts), so lanes with different rows do not diverge
+            const int wlo = max(cw, 0), whi = min(cw + WMAX - 1, nw - 1);
+            int wst[R], pre[R + 1];
+            pre[0] = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float lb = -3.0e38f, ub = 3.0e38f;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    if (bd[k].none) continue;
+                    const float e = fmaf(fb[k], static_cast<float>(r), fa[k]);
+                    if (bd[k].lower)
+                        lb = fmaxf(lb, e);
+                    else
+                        ub = fminf(ub, e);
+                }
+                const int w0 = max(static_cast<int>(ceilf(fmaxf(lb, -2.0f))), wlo);
+                const int w1 = min(static_cast<int>(floorf(fminf(ub, static_cast<float>(nw + 1)))), whi);
+                const int n = (active && r < rmax) ? max(w1 - w0 + 1, 0) : 0;
+                wst[r] = w0;
+                pre[r + 1] = pre[r] + n;
+            }
+            const int total = pre[R];
+            for (int kc = 0; kc < total; ++kc) {
+                int r = 0, base = 0, w = wst[0];
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    const bool past = kc >= pre[q];
+                    r = past ? q : r;
+                    base = past ? pre[q] : base;
+                    w = past ? wst[q] : w;
+                }
+                w += kc - base;
+                // the candidate's image values first: their LDS reads are in
+                // flight while the geometry below is computed
+                float4 pv[NG];
+                float p1 = 0.0f;
+                if constexpr (S::kQuad) {
+                    const float4* q =
+                        reinterpret_cast<const float4*>(simg) + (r * NG) * WMAX + (w - cw);
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) pv[g] = q[g * WMAX];
+                } else {
+                    p1 = simg[r * WMAX + (w - cw)];
+                }
+                const int u = u0 + r;
+                const int ix = frame ? u : w, iy = frame ? w : u;
+                const Anchor A = anch[wave * K + ((w - cw) >> 4)];
+                const int ic = min((ix & ~(kGatherTile - 1)) + kGatherTile / 2, P.nx - 1);
+                const int jc = min((iy & ~(kGatherTile - 1)) + kGatherTile / 2, P.ny - 1);
+                float f, wt;
+                Geo::pixel(Pg, A, static_cast<float>(ix - ic), static_cast<float>(iy - jc), f, wt);
+                const float fr = __builtin_amdgcn_fractf(f);
+                const int s = A.is0 + static_cast<int>(f);
+                // the forward's edge rule, evaluated without a branch
+                const bool on = (s >= 0) & ((s < P.ns - 1) | ((s == P.ns - 1) & (fr == 0.0f)));
+                wt = ((A.inside != 0) | on) ? wt : 0.0f;
+                const float w1 = wt * fr, w0 = wt - w1;
+                const float wc = s == c ? w0 : (s + 1 == c ? w1 : 0.0f);
+                if constexpr (S::kQuad) {
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        acc[4 * g] = fmaf(wc, pv[g].x, acc[4 * g]);
+                        acc[4 * g + 1] = fmaf(wc, pv[g].y, acc[4 * g + 1]);
+                        acc[4 * g + 2] = fmaf(wc, pv[g].z, acc[4 * g + 2]);
+                        acc[4 * g + 3] = fmaf(wc, pv[g].w, acc[4 * g + 3]);
+                    }
+                } else {
+                    acc[0] = fmaf(wc, p1, acc[0]);
+                }
+            }
+        }
+    }
+
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (b < nb)
+                L.proj[(static_cast<size_t>(b0 + b) * P.na + view) * P.ns + c] =
+                    static_cast<float>(static_cast<double>(acc[b]) * L.scale);
+    }
+}
+
+// Host side: workspace bytes and the launch sequence (anchor table, the
+// transposed image, the gather).  ws: caller-provided workspace of
+// gather_workspace_bytes(...) bytes, or null to allocate it on the stream.
+template <class Anchor>
+inline size_t gather_workspace_bytes(int na, int nx, int ny, int nbatch)
+{
+    const size_t ntiles = static_cast<size_t>((nx + kGatherTile - 1) / kGatherTile) *
+                          ((ny + kGatherTile - 1) / kGatherTile);
+    const size_t anc = (na * ntiles * sizeof(Anchor) + 255) & ~static_cast<size_t>(255);
+    return anc + static_cast<size_t>(nbatch) * nx * ny * sizeof(float);
+}
+
+template <class Geo, int NB, int R, int WMAX>
+int launch_gather_nb(GatherLaunch<Geo> L, hipStream_t st)
+{
+    using S = GatherShape<Geo, NB, R, WMAX>;
+    auto kern = bp_gather_t_kernel<Geo, NB, R, WMAX>;
+    if (S::kLds > 64 * 1024)
+        CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(S::kLds)));
+    dim3 grid((L.P.ns + kGatherLanes - 1) / kGatherLanes, (L.P.na + kGatherViews - 1) / kGatherViews,
+              (L.nbatch + NB - 1) / NB);
+    hipLaunchKernelGGL(kern, grid, dim3(256), S::kLds, st, L);
+    return check_launch("bp_gather_t_kernel");
+}
+
+template <class Geo>
+int launch_bp_gather_t(const typename Geo::Problem& P, const typename Geo::Tables& T,
+                       const float* img, float* proj, int nbatch, double scale, void* ws,
+                       hipStream_t st)
+{
+    using Anchor = typename Geo::Anchor;
+    GatherLaunch<Geo> L{};
+    L.P = P;
+    L.T = T;
+    L.tiles_x = (P.nx + kGatherTile - 1) / kGatherTile;
+    L.tiles_y = (P.ny + kGatherTile - 1) / kGatherTile;
+    L.nbatch = nbatch;
+    L.scale = scale;
+    L.img = img;
+    L.proj = proj;
+    const int ntiles = L.tiles_x * L.tiles_y;
+    const size_t bytes = gather_workspace_bytes<Anchor>(P.na, P.nx, P.ny, nbatch);
+    void* own = nullptr;
+    if (!ws) {
+        CTR_HIP(hipMallocAsync(&own, bytes, st));
+        ws = own;
+    }
+    auto* table = static_cast<Anchor*>(ws);
+    const size_t anc = (static_cast<size_t>(P.na) * ntiles * sizeof(Anchor) + 255) &
+                       ~static_cast<size_t>(255);
+    float* imgT = reinterpret_cast<float*>(static_cast<unsigned char*>(ws) + anc);
+    L.anchors = table;
+    L.imgT = imgT;
+    hipLaunchKernelGGL(bp_anchor_table_kernel<Geo>, dim3((ntiles + 63) / 64, P.na), dim3(64), 0,
+                       st, P, T, L.tiles_x, ntiles, table);
+    int rc = check_launch("bp_anchor_table_kernel");
+    if (rc == CTR_OK) {
+        hipLaunchKernelGGL(image_transpose_kernel, dim3((P.nx + 31) / 32, (P.ny + 31) / 32, nbatch),
+                           dim3(32, 8), 0, st, img, imgT, P.ny, P.nx);
+        rc = check_launch("image_transpose_kernel");
+    }
+    if (rc == CTR_OK) {
+        // four slices of one image must fit a 32-bit buffer range
+        const bool quad_ok = 4ull * P.nx * P.ny * 4 < (1ull << 31);
+        if (nbatch >= 16 && quad_ok)
+            rc = launch_gather_nb<Geo, 16, 4, 128>(L, st);
+        else if (nbatch >= 5 && quad_ok)
+            rc = launch_gather_nb<Geo, 8, 4, 128>(L, st);
+        else if (nbatch >= 2 && quad_ok)
+            rc = launch_gather_nb<Geo, 4, 4, 128>(L, st);
+        else
+            rc = launch_gather_nb<Geo, 1, 4, 128>(L, st);
+    }
+    if (own) CTR_HIP(hipFreeAsync(own, st));
+    return rc;
+}
+
+}  // namespace ctr

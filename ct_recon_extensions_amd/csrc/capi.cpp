@@ -24,7 +24,7 @@ int plain_fan_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int 
 int par_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
            const double* xcos, const double* ysin, hipStream_t st);
 int par_bpt(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
-            const double* xcos, const double* ysin, bool det, hipStream_t st);
+            const double* xcos, const double* ysin, bool det, void* ws, hipStream_t st);
 int par_svbp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
              const double* xcos, const double* ysin, hipStream_t st);
 int fp_rc(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch, double step,
@@ -34,6 +34,11 @@ int fp_par(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch
 int fp_par_t(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
              const double* w, const double* pos, const int* usex, unsigned int pos_stride,
              hipStream_t st);
+int fp_dd(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
+          const double* sc, const double* bo, const int* usex, hipStream_t st);
+int fp_dd_grad(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
+               const double* w, const double* pos, const int* usex, unsigned int pos_stride,
+               hipStream_t st);
 int ramp(const ctr_geom& g, int dtype, const void* in, void* out, unsigned int nrow,
          const double* filter, int weight_type, hipStream_t st);
 int fan_weighting(const ctr_geom& g, int dtype, const void* in, void* out, unsigned int nrow,
@@ -186,7 +191,7 @@ int ctr_par_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int 
     CTR_REQUIRE(xcos && ysin, CTR_ERR_ARG, "ctr_par_bp_t: null table");
     return guarded("ctr_par_bp_t", [&] {
         return ctr::par_bpt(*g, dtype, img, proj, nbatch, xcos, ysin, deterministic != 0,
-                            static_cast<hipStream_t>(stream));
+                            nullptr, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -227,6 +232,34 @@ int ctr_fp_t(const ctr_geom* g, int dtype, const void* proj, void* img, int nbat
     return guarded("ctr_fp_t", [&] {
         return ctr::fp_par_t(*g, dtype, proj, img, nbatch, weights, pos, usex, pos_stride,
                              static_cast<hipStream_t>(stream));
+    });
+}
+
+int ctr_fp_dd(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
+              const double* sincostbl, const double* beginoffset, const int* usex, void* stream)
+{
+    int rc = check_common(g, dtype, img, proj, nbatch, "ctr_fp_dd");
+    if (rc) return rc;
+    CTR_REQUIRE(sincostbl && beginoffset && usex, CTR_ERR_ARG, "ctr_fp_dd: null table");
+    return guarded("ctr_fp_dd", [&] {
+        return ctr::fp_dd(*g, dtype, img, proj, nbatch, sincostbl, beginoffset, usex,
+                          static_cast<hipStream_t>(stream));
+    });
+}
+
+int ctr_fp_dd_grad(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                   const double* weights, const double* pos, const int* usex,
+                   unsigned int pos_stride, void* stream)
+{
+    int rc = check_common(g, dtype, proj, img, nbatch, "ctr_fp_dd_grad");
+    if (rc) return rc;
+    CTR_REQUIRE(weights && pos && usex, CTR_ERR_ARG, "ctr_fp_dd_grad: null table");
+    const unsigned int need = g->nx > g->ny ? g->nx : g->ny;
+    CTR_REQUIRE(pos_stride >= need, CTR_ERR_ARG,
+                "ctr_fp_dd_grad: pos_stride must be >= max(nx, ny)");
+    return guarded("ctr_fp_dd_grad", [&] {
+        return ctr::fp_dd_grad(*g, dtype, proj, img, nbatch, weights, pos, usex, pos_stride,
+                               static_cast<hipStream_t>(stream));
     });
 }
 

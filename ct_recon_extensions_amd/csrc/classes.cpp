@@ -517,6 +517,118 @@ template class ParallelProjection2DRayDriven<double>;
 template class ParallelProjection2DRayDrivenGrad<float>;
 template class ParallelProjection2DRayDrivenGrad<double>;
 
+// ------------------------------------------------------ fp_par_2d raycasting
+bool ParallelProjection2DRayCastingPrepare::calculate_on_cpu(double* sc, double* begins,
+                                                             int* nsteps) const
+{
+    const ctr_geom g = geom_par(param_);
+    return ctr_fp_rc_tables(&g, step_size_, sc, begins, nsteps) == CTR_OK;
+}
+
+bool ParallelProjection2DRayCastingPrepare::calculate_on_gpu(double* sc, double* begins,
+                                                             int* nsteps, hipStream_t st) const
+{
+    const ctr_geom g = geom_par(param_);
+    std::vector<double> s(2 * g.na), b(2 * static_cast<size_t>(g.na) * g.ns);
+    std::vector<int> n(static_cast<size_t>(g.na) * g.ns);
+    if (ctr_fp_rc_tables(&g, step_size_, s.data(), b.data(), n.data()) != CTR_OK) return false;
+    return to_device(sc, s, st) && to_device(begins, b, st) && to_device(nsteps, n, st) &&
+           ok_sync(CTR_OK, st);
+}
+
+template <typename T>
+bool ParallelProjection2DRayCasting<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                         const double*, const int*) const
+{
+    return no_cpu("ParallelProjection2DRayCasting::calculate_on_cpu");
+}
+
+template <typename T>
+bool ParallelProjection2DRayCasting<T>::calculate_on_gpu(const T* img, T* proj, const double* sc,
+                                                         const double* begins,
+                                                         const int* nsteps, hipStream_t st) const
+{
+    const ctr_geom g = geom_par(this->param_);
+    return ok_sync(ctr_fp_rc(&g, dtype_of<T>(), img, proj, 1, step_size_, sc, begins, nsteps, st),
+                   st);
+}
+
+template class ParallelProjection2DRayCasting<float>;
+template class ParallelProjection2DRayCasting<double>;
+
+// ---------------------------------------------------- fp_par_2d disdriven
+bool ParallelProjection2DDisDrivenPrep::calculate_on_cpu(double* sc, double* bo, int* usex) const
+{
+    const ctr_geom g = geom_par(param_);
+    return ctr_fp_dd_tables(&g, sc, bo, usex) == CTR_OK;
+}
+
+bool ParallelProjection2DDisDrivenPrep::calculate_on_gpu(double* sc, double* bo, int* usex,
+                                                         hipStream_t st) const
+{
+    const ctr_geom g = geom_par(param_);
+    std::vector<double> s(2 * g.na), b(3 * static_cast<size_t>(g.na) * g.ns);
+    std::vector<int> u(g.na);
+    if (ctr_fp_dd_tables(&g, s.data(), b.data(), u.data()) != CTR_OK) return false;
+    return to_device(sc, s, st) && to_device(bo, b, st) && to_device(usex, u, st) &&
+           ok_sync(CTR_OK, st);
+}
+
+bool ParallelProjection2DDisDrivenGradPrep::calculate_on_cpu(double* w, double* pos,
+                                                             int* usex) const
+{
+    const ctr_geom g = geom_par(param_);
+    return ctr_fp_grad_tables(&g, w, pos, usex, g.nx) == CTR_OK;
+}
+
+bool ParallelProjection2DDisDrivenGradPrep::calculate_on_gpu(double* w, double* pos, int* usex,
+                                                             hipStream_t st) const
+{
+    const ctr_geom g = geom_par(param_);
+    std::vector<double> wv(g.na), pv(2 * static_cast<size_t>(g.na) * g.nx);
+    std::vector<int> u(g.na);
+    if (ctr_fp_grad_tables(&g, wv.data(), pv.data(), u.data(), g.nx) != CTR_OK) return false;
+    return to_device(w, wv, st) && to_device(pos, pv, st) && to_device(usex, u, st) &&
+           ok_sync(CTR_OK, st);
+}
+
+template <typename T>
+bool ParallelProjection2DDisDriven<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                        const double*, const int*) const
+{
+    return no_cpu("ParallelProjection2DDisDriven::calculate_on_cpu");
+}
+
+template <typename T>
+bool ParallelProjection2DDisDriven<T>::calculate_on_gpu(const T* img, T* proj, const double* sc,
+                                                        const double* bo, const int* usex,
+                                                        hipStream_t st) const
+{
+    const ctr_geom g = geom_par(this->param_);
+    return ok_sync(ctr_fp_dd(&g, dtype_of<T>(), img, proj, 1, sc, bo, usex, st), st);
+}
+
+template <typename T>
+bool ParallelProjection2DDisDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
+                                                            const double*, const int*) const
+{
+    return no_cpu("ParallelProjection2DDisDrivenGrad::calculate_on_cpu");
+}
+
+template <typename T>
+bool ParallelProjection2DDisDrivenGrad<T>::calculate_on_gpu(const T* proj, T* img,
+                                                            const double* w, const double* pos,
+                                                            const int* usex, hipStream_t st) const
+{
+    const ctr_geom g = geom_par(this->param_);
+    return ok_sync(ctr_fp_dd_grad(&g, dtype_of<T>(), proj, img, 1, w, pos, usex, g.nx, st), st);
+}
+
+template class ParallelProjection2DDisDriven<float>;
+template class ParallelProjection2DDisDriven<double>;
+template class ParallelProjection2DDisDrivenGrad<float>;
+template class ParallelProjection2DDisDrivenGrad<double>;
+
 // ----------------------------------------------------------------- filter
 bool RampFilterPrep::calculate_on_cpu(double* filter) const
 {

@@ -3,14 +3,15 @@
 // baseline.  The forward kernel and the transpose are the BP engine
 // (bp_engine.hpp) with the fan geometry policy (fan_geom.hpp): workgroups of
 // one pixel tile x NB slices sharing the geometry, LDS-staged detector spans,
-// FP64 per-(tile, view) anchors and FP32 pixel offsets; the transpose is a
-// view-parallel scatter into LDS rows with an order-independent fixed-point
-// mode.
+// FP64 per-(tile, view) anchors and FP32 pixel offsets; the transpose gathers
+// each (view, channel) output in registers from the pixels that reach it
+// (bp_gather.hpp), with the forward's anchors and pixel geometry.
 //
 // Edge rule (src/bp_fan_2d.cc:72-80): a pixel-view contributes iff
 // 0 <= s <= ns-1, with taps floor(s) and ceil(s).
 #include "ctr_common.hpp"
 #include "bp_engine.hpp"
+#include "bp_gather.hpp"
 #include "fan_geom.hpp"
 #include "tile_common.hpp"
 
@@ -313,34 +314,9 @@ int fan_bp_f32(const ctr_geom& g, const float* proj, float* img, int nbatch, con
 #undef CTR_FAN_NB
 }
 
-template <bool EXACT, int NB, bool DET>
-static int launch_fan_bpt(const ctr_geom& g, const FanProblem& P, int nbatch, double scale,
-                          const double* fixs, const float* img, float* proj, const FanTables& T,
-                          hipStream_t st)
-{
-    constexpr int TW = 16, TH = 16;
-    BptLaunch<FanGeo<EXACT>> L{};
-    L.P = P;
-    L.nbatch = nbatch;
-    L.tiles_x = div_up(g.nx, TW);
-    L.tiles_y = div_up(g.ny, TH);
-    L.scale = scale;
-    L.fix_scale = fixs;
-    const size_t lds = kAnchorChunk * sizeof(FanAnchor) +
-                       static_cast<size_t>(NB) * (g.ns + 1) * (DET ? 8 : 4);
-    CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "fan BP transpose: ns too large for LDS");
-    auto kern = bp_transpose_kernel<FanGeo<EXACT>, TW, TH, NB, DET>;
-    if (lds > 64 * 1024)
-        CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    static_cast<int>(lds)));
-    hipLaunchKernelGGL(kern, dim3(g.na, div_up(nbatch, NB)), dim3(256), lds, st, img, proj, T, L);
-    return check_launch("bp_transpose_kernel<fan>");
-}
-
-// Deterministic mode needs fix_workspace_bytes(nbatch) bytes of device
-// workspace for the per-slice fixed-point scales: `ws` when the caller
-// brings it, else a stream-ordered allocation freed after the launch.
+// float32 transpose: the gather form (bp_gather.hpp), deterministic in both
+// modes.  ws: caller workspace of gather_workspace_bytes<FanAnchor>(...)
+// bytes, or null for a stream-ordered allocation.
 int fan_bpt_f32(const ctr_geom& g, const float* img, float* proj, int nbatch, const double* xpos,
                 const double* ypos, const double* sc, double scale, bool det, void* ws,
                 hipStream_t st)
@@ -350,40 +326,10 @@ int fan_bpt_f32(const ctr_geom& g, const float* img, float* proj, int nbatch, co
     bool exact = false;
     int rc = fan_problem(g, 16, 16, P, lmax, exact);
     if (rc) return rc;
+    (void)det;
     const FanTables T{xpos, ypos, sc};
-    if (det) {
-        // Per slice: 2^62 / (max|img| * max weight * 2 nx ny contributions per channel).
-        void* own = nullptr;
-        if (!ws) {
-            CTR_HIP(hipMallocAsync(&own, fix_workspace_bytes(nbatch), st));
-            ws = own;
-        }
-        const double wmax = g.dsd / fan_min_distance(g, 16, 16);
-        const double bound = 2.0 * g.nx * g.ny * std::max(1.0, wmax * wmax);
-        hipError_t e = launch_fix_scales(img, nbatch, static_cast<size_t>(g.nx) * g.ny, bound, ws,
-                                         st);
-        if (e != hipSuccess) {
-            set_error(std::string("fix_scale_kernel: ") + hipGetErrorString(e));
-            rc = CTR_ERR_HIP;
-        }
-        const double* fs = static_cast<const double*>(ws);
-        if (rc == CTR_OK)
-            rc = exact ? launch_fan_bpt<true, 4, true>(g, P, nbatch, scale, fs, img, proj, T, st)
-                       : launch_fan_bpt<false, 4, true>(g, P, nbatch, scale, fs, img, proj, T, st);
-        if (own) CTR_HIP(hipFreeAsync(own, st));
-        return rc;
-    }
-    const int nb = nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
-#define CTR_FANT(NB_)                                                                           \
-    return exact ? launch_fan_bpt<true, NB_, false>(g, P, nbatch, scale, nullptr, img, proj, T, st) \
-                 : launch_fan_bpt<false, NB_, false>(g, P, nbatch, scale, nullptr, img, proj, T, st);
-    switch (nb) {
-        case 1: CTR_FANT(1)
-        case 2: CTR_FANT(2)
-        case 4: CTR_FANT(4)
-        default: CTR_FANT(8)
-    }
-#undef CTR_FANT
+    return exact ? launch_bp_gather_t<FanGeo<true>>(P, T, img, proj, nbatch, scale, ws, st)
+                 : launch_bp_gather_t<FanGeo<false>>(P, T, img, proj, nbatch, scale, ws, st);
 }
 
 int fan_bp_f64(const ctr_geom& g, const double* proj, double* img, int nbatch,

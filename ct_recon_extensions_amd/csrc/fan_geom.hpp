@@ -155,6 +155,26 @@ struct FanGeo {
     {
         asm volatile("" : "+v"(P.p1), "+v"(P.g1));
     }
+    // Level line s = s_star in pixel-index coordinates: the ray at fan angle
+    // g* = (s_star - cents) ds / dsd, r - tan(g*) d = 0, i.e.
+    // L(ix, iy) = pi ix + qj iy + rr >= 0 exactly where s >= s_star for every
+    // pixel in front of the source (d > 0; the host admits no other).  g* is
+    // clamped to +-1.47 rad: every admitted pixel has |g| <= asin(0.95), so a
+    // clamped line still bounds the same set of image pixels.
+    __device__ static void boundary(const Problem& P, const Tables& T, int view, double s_star,
+                                    double& pi, double& qj, double& rr)
+    {
+        const double sb = T.sincos[2 * view], cb = T.sincos[2 * view + 1];
+        const double g = fmin(fmax((s_star - P.cents) * P.ds / P.dsd, -1.47), 1.47);
+        const double t = tan(g);
+        const double pp = cb - t * sb, qq = sb + t * cb, r0 = -t * P.dso;
+        const double x0 = T.xpos[0], y0 = T.ypos[0];
+        const double xs = P.nx > 1 ? (T.xpos[P.nx - 1] - x0) / (P.nx - 1) : P.dx;
+        const double ys = P.ny > 1 ? (T.ypos[P.ny - 1] - y0) / (P.ny - 1) : -P.dy;
+        pi = pp * xs;
+        qj = qq * ys;
+        rr = pp * x0 + qq * y0 + r0;
+    }
 };
 
 }  // namespace ctr

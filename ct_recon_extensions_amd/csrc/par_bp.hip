@@ -8,6 +8,7 @@
 // from the reference's own FP64 tables xcos / ysin and the slopes from their
 // differences.  Output scale is the signed orbit (src/bp_par_2d.cc:83).
 #include "bp_engine.hpp"
+#include "bp_gather.hpp"
 #include "ctr_common.hpp"
 #include "tile_common.hpp"
 
@@ -76,6 +77,18 @@ struct ParGeo {
     {
         f = fmaf(A.a, fi, fmaf(A.b, fj, A.fc));
         w = 1.0f;
+    }
+    // Level line s = s_star in pixel-index coordinates: the tables are linear
+    // in the index (xcos[ix][view] = x0 cos + ix dx cos), so
+    // L(ix, iy) = pi ix + qj iy + rr = ds (s - s_star) >= 0 where s >= s_star.
+    __device__ static void boundary(const Problem& P, const Tables& T, int view, double s_star,
+                                    double& pi, double& qj, double& rr)
+    {
+        const size_t na = P.na;
+        const double xc0 = T.xcos[view], ys0 = T.ysin[view];
+        pi = P.nx > 1 ? (T.xcos[(P.nx - 1) * na + view] - xc0) / (P.nx - 1) : 0.0;
+        qj = P.ny > 1 ? (T.ysin[(P.ny - 1) * na + view] - ys0) / (P.ny - 1) : 0.0;
+        rr = xc0 + ys0 - (s_star - P.cents) * P.ds;
     }
 };
 
@@ -215,22 +228,6 @@ int launch_par_bp(const BpLaunch<ParGeo>& L, const ParTables& T, hipStream_t st)
     return CTR_ERR_SHAPE;
 }
 
-template <int NB, bool DET>
-int launch_par_bpt(const BptLaunch<ParGeo>& L, int ns, int na, const float* img, float* proj,
-                   const ParTables& T, hipStream_t st)
-{
-    const size_t lds = kAnchorChunk * sizeof(ParAnchor) + static_cast<size_t>(NB) * (ns + 1) *
-                       (DET ? 8 : 4);
-    CTR_REQUIRE(lds <= 160 * 1024, CTR_ERR_SHAPE, "parallel BP transpose: ns too large for LDS");
-    auto kern = bp_transpose_kernel<ParGeo, 16, 16, NB, DET>;
-    if (lds > 64 * 1024)
-        CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    static_cast<int>(lds)));
-    hipLaunchKernelGGL(kern, dim3(na, div_up(L.nbatch, NB)), dim3(256), lds, st, img, proj, T, L);
-    return check_launch("bp_transpose_kernel<par>");
-}
-
 }  // namespace
 
 int par_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
@@ -266,51 +263,24 @@ int par_bp(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch
 }
 
 int par_bpt(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
-            const double* xcos, const double* ysin, bool det, hipStream_t st)
+            const double* xcos, const double* ysin, bool det, void* ws, hipStream_t st)
 {
-    constexpr int TW = 16, TH = 16;
-    BptLaunch<ParGeo> L{};
+    ParProblem P{};
     int lmax = 0;
-    int rc = par_problem(g, TW, TH, L.P, lmax);
+    int rc = par_problem(g, 16, 16, P, lmax);
     if (rc) return rc;
-    L.nbatch = nbatch;
-    L.tiles_x = div_up(g.nx, TW);
-    L.tiles_y = div_up(g.ny, TH);
-    L.scale = g.orbit;
+    // Both forms below sum in a fixed order, so `det` needs nothing extra.
+    (void)det;
     if (dtype == CTR_F64) {
-        // the gather form: deterministic in both modes
         hipLaunchKernelGGL(par_bpt_f64_gather_kernel,
                            dim3(div_up(g.ns, 64), g.na, div_up(nbatch, kParGatherSlices)),
                            dim3(64), 0, st, static_cast<const double*>(img),
-                           static_cast<double*>(proj), xcos, ysin, L.P, nbatch, g.orbit);
+                           static_cast<double*>(proj), xcos, ysin, P, nbatch, g.orbit);
         return check_launch("par_bpt_f64_gather_kernel");
     }
     const ParTables T{xcos, ysin};
-    auto im = static_cast<const float*>(img);
-    auto pr = static_cast<float*>(proj);
-    const int ns = static_cast<int>(g.ns), na = static_cast<int>(g.na);
-    if (det) {
-        // per-slice fixed-point scales: 2^62 / (max|img| * 2 nx ny)
-        void* ws = nullptr;
-        CTR_HIP(hipMallocAsync(&ws, fix_workspace_bytes(nbatch), st));
-        hipError_t e = launch_fix_scales(im, nbatch, static_cast<size_t>(g.nx) * g.ny,
-                                         2.0 * g.nx * g.ny, ws, st);
-        if (e != hipSuccess) {
-            set_error(std::string("fix_scale_kernel: ") + hipGetErrorString(e));
-            rc = CTR_ERR_HIP;
-        }
-        L.fix_scale = static_cast<const double*>(ws);
-        if (rc == CTR_OK) rc = launch_par_bpt<4, true>(L, ns, na, im, pr, T, st);
-        CTR_HIP(hipFreeAsync(ws, st));
-        return rc;
-    }
-    const int nb = nbatch >= 8 ? 8 : nbatch >= 4 ? 4 : nbatch >= 2 ? 2 : 1;
-    switch (nb) {
-        case 1: return launch_par_bpt<1, false>(L, ns, na, im, pr, T, st);
-        case 2: return launch_par_bpt<2, false>(L, ns, na, im, pr, T, st);
-        case 4: return launch_par_bpt<4, false>(L, ns, na, im, pr, T, st);
-        default: return launch_par_bpt<8, false>(L, ns, na, im, pr, T, st);
-    }
+    return launch_bp_gather_t<ParGeo>(P, T, static_cast<const float*>(img),
+                                      static_cast<float*>(proj), nbatch, g.orbit, ws, st);
 }
 
 }  // namespace ctr

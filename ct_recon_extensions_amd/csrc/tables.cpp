@@ -188,6 +188,49 @@ int ctr_fp_tables(const ctr_geom* g, double* sincostbl, double* beginoffset, int
     return CTR_OK;
 }
 
+// Distance-driven FP tables (src/fp_par_2d.cc:396-457): the ray-driven
+// tables' angle wrap, branch rule, per-channel begin (accumulated) and step,
+// plus half the channel's footprint width in pixel units:
+// beginoffset[na][ns][3] = (begin, step, chstep / 2).
+int ctr_fp_dd_tables(const ctr_geom* g, double* sincostbl, double* beginoffset, int* usex)
+{
+    CTR_REQUIRE(g && sincostbl && beginoffset && usex, CTR_ERR_ARG,
+                "ctr_fp_dd_tables: null argument");
+    const double cx = cent_x(*g), cy = cent_y(*g), cs = cent_s(*g);
+    const double q = M_PI / 4;
+    double angle = g->orbit_start;
+    double* bo = beginoffset;
+    for (unsigned int a = 0; a < g->na; ++a) {
+        while (angle < 0) angle += 2 * M_PI;
+        while (angle >= 2 * M_PI) angle -= 2 * M_PI;
+        double sa, ca;
+        sin_cos(angle, sa, ca);
+        sincostbl[2 * a] = sa;
+        sincostbl[2 * a + 1] = ca;
+        const bool ybranch = (angle >= q && angle < 3 * q) || (angle >= 5 * q && angle < 7 * q);
+        usex[a] = ybranch ? 0 : 1;
+        double step, chstep, begin;
+        if (!ybranch) {
+            step = g->dy * sa / (ca * g->dx);
+            chstep = g->ds / (ca * g->dx);
+            begin = cx - cy * step - cs * chstep;
+        } else {
+            step = g->dx * ca / (sa * g->dy);
+            chstep = -g->ds / (sa * g->dy);
+            begin = cy - cx * step - cs * chstep;
+        }
+        for (unsigned int s = 0; s < g->ns; ++s) {
+            bo[0] = begin;
+            bo[1] = step;
+            bo[2] = chstep / 2;
+            begin += chstep;
+            bo += 3;
+        }
+        angle += g->orbit;
+    }
+    return CTR_OK;
+}
+
 // FP transpose tables (src/fp_par_2d.cc:285-330); no angle wrap, as in the
 // reference.  pos is [na][pos_stride][2].
 int ctr_fp_grad_tables(const ctr_geom* g, double* weights, double* pos, int* usex,

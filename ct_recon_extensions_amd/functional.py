@@ -332,16 +332,58 @@ class _FPT(torch.autograd.Function):
         return _FP.apply(g.contiguous(), ctx.p, ctx.t_fp, ctx.t_grad), None, None, None
 
 
+class _FPDD(torch.autograd.Function):
+    """Distance-driven FP; backward: the reference's distance-driven gradient
+    (its TF op registers that one for method "disdriven")."""
+
+    @staticmethod
+    def forward(ctx, img3, p, t_fp, t_grad):
+        ctx.p, ctx.t_fp, ctx.t_grad = p, t_fp, t_grad
+        n = img3.shape[0]
+        out = torch.empty((n, p.na, p.ns), dtype=img3.dtype, device=img3.device)
+        sc, bo, usex = t_fp
+        _call_batched("ctr_fp_dd", n, C.byref(p.ctr()), _dtype_code(img3), _ptr(img3),
+                      _ptr(out), n, _ptr(sc), _ptr(bo), _ptr(usex), _stream(img3))
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        return _FPDDGrad.apply(g.contiguous(), ctx.p, ctx.t_fp, ctx.t_grad), None, None, None
+
+
+class _FPDDGrad(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, proj3, p, t_fp, t_grad):
+        ctx.p, ctx.t_fp, ctx.t_grad = p, t_fp, t_grad
+        n = proj3.shape[0]
+        out = torch.empty((n, p.ny, p.nx), dtype=proj3.dtype, device=proj3.device)
+        w, pos, usex = t_grad
+        _call_batched("ctr_fp_dd_grad", n, C.byref(p.ctr()), _dtype_code(proj3), _ptr(proj3),
+                      _ptr(out), n, _ptr(w), _ptr(pos), _ptr(usex), max(p.nx, p.ny),
+                      _stream(proj3))
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        # the reference defines no gradient of its gradient op; the
+        # distance-driven FP is the pair's other half
+        return _FPDD.apply(g.contiguous(), ctx.p, ctx.t_fp, ctx.t_grad), None, None, None
+
+
 def parallel_projection_2d(img, img_shape, img_space, proj_shape, channel_space,
                            img_offset=(0, 0), channel_offset=0, orbit_start=0,
                            orbit=math.pi / 180, fov=-1, method="raydriven"):
     """Parallel FP (python/tf_ctrecon.py:173-195).  img [N, ny, nx, 1] ->
     proj [N, na, ns, 1].  method "raydriven" (src/fp_par_2d.cc:210-280;
-    backward: the gather-form transpose, src/fp_par_2d.cc:332-391) or
+    backward: the gather-form transpose, src/fp_par_2d.cc:332-391),
     "raycasting" (src/fp_par_2d.cc:18-144: rays from their entry points on
     the FOV circle of radius fov, step 0.125, bilinear samples; bit-identical
-    to the reference; like the reference it has no gradient)."""
-    _check_method(method, ("raydriven", "raycasting"), "parallel_projection_2d")
+    to the reference; like the reference it has no gradient) or "disdriven"
+    (src/fp_par_2d.cc:396-543: channel footprints averaging the image per
+    row or column; backward: the reference's distance-driven gradient,
+    src/fp_par_2d.cc:548-655, which approximates the transpose; both
+    bit-identical to the reference)."""
+    _check_method(method, ("raydriven", "raycasting", "disdriven"), "parallel_projection_2d")
     p = _par_param(img_shape, img_space, proj_shape, channel_space, img_offset, channel_offset,
                    orbit_start, orbit, fov, cls=ParallelProjection2DParam)
     x, restore = _to_batch(img, p.ny, p.nx, "img")
@@ -352,6 +394,9 @@ def parallel_projection_2d(img, img_shape, img_space, proj_shape, channel_space,
         _call_batched("ctr_fp_rc", n, C.byref(p.ctr()), _dtype_code(x), _ptr(x), _ptr(out), n,
                       RC_STEP, _ptr(sc), _ptr(begins), _ptr(nsteps), _stream(x))
         return restore(out)
+    if method == "disdriven":
+        return restore(_FPDD.apply(x, p, device_tables("fp_dd", p, x.device),
+                                   device_tables("fp_grad", p, x.device)))
     return restore(_FP.apply(x, p, device_tables("fp", p, x.device),
                              device_tables("fp_grad", p, x.device)))
 
@@ -362,10 +407,13 @@ def parallel_projection_2d_grad(grad_in, img_shape, img_space, proj_shape, chann
     """Transpose of parallel_projection_2d (tensorflow/fp_par_2d_ops.cu:245-374)."""
     if method == "raycasting":  # tensorflow/fp_par_2d_ops.cu:287-291
         raise RuntimeError("Gradient is not yet implemented for raycasting forward projector.")
-    _check_method(method, ("raydriven",), "parallel_projection_2d_grad")
+    _check_method(method, ("raydriven", "disdriven"), "parallel_projection_2d_grad")
     p = _par_param(img_shape, img_space, proj_shape, channel_space, img_offset, channel_offset,
                    orbit_start, orbit, fov, cls=ParallelProjection2DParam)
     x, restore = _to_batch(grad_in, p.na, p.ns, "grad_in")
+    if method == "disdriven":
+        return restore(_FPDDGrad.apply(x, p, device_tables("fp_dd", p, x.device),
+                                       device_tables("fp_grad", p, x.device)))
     return restore(_FPT.apply(x, p, device_tables("fp", p, x.device),
                               device_tables("fp_grad", p, x.device)))
 

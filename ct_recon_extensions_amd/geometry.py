@@ -168,6 +168,13 @@ def fp_tables(p: ParallelProjection2DParam):
 RC_STEP = 0.125  # the reference's ray-casting step (include/fp_par_2d.h:160,186)
 
 
+def fp_dd_tables(p: ParallelProjection2DParam):
+    sc, bo = np.zeros(2 * p.na), np.zeros(3 * p.na * p.ns)
+    usex = np.zeros(p.na, np.int32)
+    _lib.call("ctr_fp_dd_tables", C.byref(p.ctr()), _ptr(sc), _ptr(bo), _ptr(usex))
+    return sc, bo, usex
+
+
 def fp_rc_tables(p: ParallelProjection2DParam, step: float = RC_STEP):
     sc, begins = np.zeros(2 * p.na), np.zeros(2 * p.na * p.ns)
     nsteps = np.zeros(p.na * p.ns, np.int32)
@@ -224,6 +231,8 @@ def device_tables(kind: str, p, device, angles=None):
         host = fp_tables(p)
     elif kind == "fp_rc":
         host = fp_rc_tables(p)
+    elif kind == "fp_dd":
+        host = fp_dd_tables(p)
     elif kind == "fp_grad":
         host = fp_grad_tables(p)
     elif kind == "ramp":

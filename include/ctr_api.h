@@ -87,10 +87,16 @@ int ctr_fp_tables(const ctr_geom* g, double* sincostbl, double* beginoffset, int
  * (0 for rays missing the circle). */
 int ctr_fp_rc_tables(const ctr_geom* g, double step, double* sincostbl, double* begins,
                      int* nsteps);
-/* ParallelProjection2DRayDrivenGradPrep (src/fp_par_2d.cc:285-330):
- * weights[na], pos[na][pos_stride][2], usex[na].  The reference uses
- * pos_stride == nx (and so requires nx == ny); pos_stride >= max(nx, ny)
- * lifts that restriction. */
+/* ParallelProjection2DDisDrivenPrep (src/fp_par_2d.cc:396-457):
+ * sincostbl[2na], beginoffset[na][ns][3] = (footprint centre on the first
+ * row or column, its shift per row or column, half its width; pixel units),
+ * usex[na]. */
+int ctr_fp_dd_tables(const ctr_geom* g, double* sincostbl, double* beginoffset, int* usex);
+/* ParallelProjection2DRayDrivenGradPrep (src/fp_par_2d.cc:285-330), and
+ * ParallelProjection2DDisDrivenGradPrep (src/fp_par_2d.cc:548-599), which
+ * computes the same tables: weights[na], pos[na][pos_stride][2], usex[na].
+ * The reference uses pos_stride == nx (and so requires nx == ny);
+ * pos_stride >= max(nx, ny) lifts that restriction. */
 int ctr_fp_grad_tables(const ctr_geom* g, double* weights, double* pos, int* usex,
                        unsigned int pos_stride);
 /* RampFilterPrep (src/filter.cc:18-65): filter[2ns+1], tap n at [ns+n]. */
@@ -173,6 +179,25 @@ int ctr_fp_rc(const ctr_geom* g, int dtype, const void* img, void* proj, int nba
 int ctr_fp_t(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
              const double* weights, const double* pos, const int* usex, unsigned int pos_stride,
              void* stream);
+/* Distance-driven parallel FP, ParallelProjection2DDisDriven<T>
+ * (src/fp_par_2d.cc:459-543; TF FP op with method "disdriven",
+ * tensorflow/fp_par_2d_ops.cu:175-186): img [nbatch][ny][nx] -> proj
+ * [nbatch][na][ns] from the tables of ctr_fp_dd_tables; bit-identical to the
+ * reference's CPU path, including its unsigned-index rule (a row whose
+ * footprint starts left of the first pixel adds nothing). */
+int ctr_fp_dd(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
+              const double* sincostbl, const double* beginoffset, const int* usex,
+              void* stream);
+/* The gradient the reference defines for it, ParallelProjection2DDisDrivenGrad<T>
+ * (src/fp_par_2d.cc:601-655; the TF op's gradient for "disdriven",
+ * tensorflow/fp_par_2d_ops.cu:304-315): proj [nbatch][na][ns] -> img
+ * [nbatch][ny][nx] from the tables of ctr_fp_grad_tables.  Bit-identical to
+ * the reference where it reads inside the detector (channels past ns - 1,
+ * which the reference reads beyond the row, are not read).  Not an exact
+ * transpose of ctr_fp_dd: the pair is the reference's. */
+int ctr_fp_dd_grad(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                   const double* weights, const double* pos, const int* usex,
+                   unsigned int pos_stride, void* stream);
 /* Ramp filter, RampFilter<T> / RampFilterGrad<T> (src/filter.cc:67-124):
  * out[r][is] = ds * sum_k in[r][k] h[is-k] over nrow_total rows of ns.
  * filter: device copy of the 2ns+1 FP64 taps.  Optional fused fan weighting

@@ -580,6 +580,157 @@ DEF_FP_GRAD(float, f)
 DEF_FP_GRAD(double, d)
 
 /* ------------------------------------------------------------------ */
+/* Distance-driven parallel FP tables, src/fp_par_2d.cc:396-457: per view the
+ * angle wrapped into [0, 2pi), (sin, cos), the branch (x-branch outside
+ * [pi/4, 3pi/4) and [5pi/4, 7pi/4)), and per channel (begin, offset,
+ * half-width): the channel's footprint centre on the first row (x-branch) or
+ * column (y-branch) in pixel units, accumulated channel by channel, its
+ * shift per row or column, and half its width. */
+int orc_fp_dd_prep(const orc_geom* g, double* sincostbl, double* bo, int* usex)
+{
+    const double centx = (double)(g->nx - 1) / 2.0 + g->offset_x;
+    const double centy = (double)(g->ny - 1) / 2.0 + g->offset_y;
+    const double cents = (double)(g->ns - 1) / 2.0 + g->offset_s;
+    double angle = g->orbit_start;
+    unsigned int ia, is;
+    double* bp = bo;
+    for (ia = 0; ia < g->na; ++ia) {
+        double sa, ca, off, off2, begin;
+        while (angle < 0) angle += 2 * M_PI;
+        while (angle >= 2 * M_PI) angle -= 2 * M_PI;
+        sincos(angle, &sa, &ca);
+        sincostbl[2 * ia] = sa;
+        sincostbl[2 * ia + 1] = ca;
+        usex[ia] = !((angle >= M_PI_4 && angle < 3 * M_PI_4) ||
+                     (angle >= 5 * M_PI_4 && angle < 7 * M_PI_4));
+        if (usex[ia]) {
+            off = g->dy * sa / (ca * g->dx);
+            off2 = g->ds / (ca * g->dx);
+            begin = centx - centy * off - cents * off2;
+        } else {
+            off = g->dx * ca / (sa * g->dy);
+            off2 = -g->ds / (sa * g->dy);
+            begin = centy - centx * off - cents * off2;
+        }
+        for (is = 0; is < g->ns; ++is) {
+            bp[0] = begin;
+            bp[1] = off;
+            bp[2] = off2 / 2;
+            begin += off2;
+            bp += 3;
+        }
+        angle += g->orbit;
+    }
+    return 1;
+}
+
+/* Distance-driven parallel FP, src/fp_par_2d.cc:459-543.  Per channel and
+ * per row (x-branch; per column for the y-branch) the footprint
+ * [left, right] = centre -+ half-width, shifted by `offset` from row to row,
+ * averages the cell values (img[v] + img[v+1]) / 2 of the cells v it
+ * overlaps, weighted by the overlap length; the row averages are summed and
+ * scaled by |dy/cos| (|dx/sin|).  Arithmetic as the reference's: each term
+ * in double, the sums tsum, lsum and sum in T.  The reference's cell index
+ * is unsigned: a footprint starting left of the first cell turns
+ * floor(left) < 0 into a huge index, the loop does not run and the row adds
+ * nothing (x86-64 g++ build); that rule is kept.  Cells v >= n-1 are
+ * skipped (the pair v, v+1 must exist). */
+#define DEF_FP_DD(T, SUF)                                                         \
+int orc_fp_dd_##SUF(const orc_geom* g, const T* img, T* proj, const double* sc,   \
+                    const double* bo, const int* usex)                            \
+{                                                                                 \
+    unsigned int ia, is, u, v;                                                    \
+    for (ia = 0; ia < g->na; ++ia) {                                              \
+        const int bx = usex[ia];                                                  \
+        const double dw = bx ? fabs(g->dy / sc[2 * ia + 1]) : fabs(g->dx / sc[2 * ia]); \
+        const unsigned int nu = bx ? g->ny : g->nx, nv = bx ? g->nx : g->ny;     \
+        const size_t su = bx ? g->nx : 1, sv = bx ? 1 : g->nx;                   \
+        for (is = 0; is < g->ns; ++is) {                                          \
+            const double* b = bo + 3 * ((size_t)ia * g->ns + is);                 \
+            double left = b[0] - b[2], right = b[0] + b[2];                       \
+            const double off = b[1];                                              \
+            T sum = 0;                                                            \
+            if (left > right) { double t = left; left = right; right = t; }       \
+            for (u = 0; u < nu; ++u) {                                            \
+                T tsum = 0, lsum = 0;                                             \
+                const double fl = floor(left);                                    \
+                if (fl >= 0) {                                                    \
+                    for (v = (unsigned int)fl; v < ceil(right); ++v) {            \
+                        if (v >= nv - 1) continue;                                \
+                        const double len = fmin((double)(v + 1), right) -         \
+                                           fmax((double)v, left);                 \
+                        tsum += len * (img[u * su + v * sv] +                     \
+                                       img[u * su + (v + 1) * sv]) / 2;           \
+                        lsum += len;                                              \
+                    }                                                             \
+                }                                                                 \
+                if (lsum > 0) sum += tsum / lsum;                                 \
+                left += off;                                                      \
+                right += off;                                                     \
+            }                                                                     \
+            proj[(size_t)ia * g->ns + is] = (T)(sum * dw);                        \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_FP_DD(float, f)
+DEF_FP_DD(double, d)
+
+/* Gradient of the distance-driven FP as the reference defines it,
+ * src/fp_par_2d.cc:601-655 (the TF op's gradient for method "disdriven").
+ * Its tables (src/fp_par_2d.cc:548-599) are those of orc_fp_grad_prep:
+ * weights |dy/cos| or |dx/sin| and per row or column (begin, o1).  Per pixel
+ * and view: mid = begin + o1 * i along the branch's axis, the span
+ * [mid - o1, mid + o1] (one-sided at the first and last pixel), and every
+ * channel is whose cell [is - 1/2, is + 1/2] it overlaps adds
+ * overlap * proj[is] / 2; sums in double.  Like the reference's unsigned
+ * channel index, a span with ceil(left - 1/2) < 0 adds nothing (kept);
+ * channels past ns - 1, which the reference reads beyond the row, are not
+ * read here. */
+#define DEF_FP_DD_GRAD(T, SUF)                                                    \
+int orc_fp_dd_grad_##SUF(const orc_geom* g, const T* proj, T* img,                \
+                         const double* weights, const double* pos, const int* usex) \
+{                                                                                 \
+    unsigned int ix, iy, ia, is;                                                  \
+    for (iy = 0; iy < g->ny; ++iy) {                                              \
+        for (ix = 0; ix < g->nx; ++ix) {                                          \
+            double sum = 0.0;                                                     \
+            for (ia = 0; ia < g->na; ++ia) {                                      \
+                const T* pr = proj + (size_t)ia * g->ns;                          \
+                double begin, off, mid, left, right, tmp = 0.0, c0;               \
+                if (usex[ia]) {                                                   \
+                    const double* pp = pos + 2 * ((size_t)ia * g->nx + iy);       \
+                    begin = pp[0]; off = pp[1];                                   \
+                    mid = begin + off * ix;                                       \
+                    left = ix == 0 ? mid : mid - off;                             \
+                    right = ix == g->nx - 1 ? mid : mid + off;                    \
+                } else {                                                          \
+                    const double* pp = pos + 2 * ((size_t)ia * g->nx + ix);       \
+                    begin = pp[0]; off = pp[1];                                   \
+                    mid = begin + off * iy;                                       \
+                    left = iy == 0 ? mid : mid - off;                             \
+                    right = iy == g->ny - 1 ? mid : mid + off;                    \
+                }                                                                 \
+                if (left > right) { double t = left; left = right; right = t; }   \
+                c0 = ceil(left - 0.5);                                            \
+                if (c0 >= 0) {                                                    \
+                    for (is = (unsigned int)c0; is <= floor(right + 0.5) && is < g->ns; ++is) { \
+                        const double len = fmin((double)is + 0.5, right) -        \
+                                           fmax((double)is - 0.5, left);          \
+                        tmp += len * pr[is] / 2;                                  \
+                    }                                                             \
+                }                                                                 \
+                sum += tmp * weights[ia];                                         \
+            }                                                                     \
+            img[ix + iy * g->nx] = (T)sum;                                        \
+        }                                                                         \
+    }                                                                             \
+    return 1;                                                                     \
+}
+DEF_FP_DD_GRAD(float, f)
+DEF_FP_DD_GRAD(double, d)
+
+/* ------------------------------------------------------------------ */
 /* Ramp filter taps filter[ns + n], n in [-ns, ns], src/filter.cc:18-65.
  * Returns 0 for an unknown type (:62). */
 int orc_ramp_prep(const orc_geom* g, double* filter)

@@ -233,6 +233,43 @@ def fp_grad(g: Geom, proj: np.ndarray, tables, kind="oracle"):
     return img
 
 
+# -------------------------------------------------- distance-driven FP
+def fp_dd_prep(g: Geom, kind="oracle"):
+    """Distance-driven FP tables (src/fp_par_2d.cc:396-457): sincos [na][2],
+    (begin, offset, half-width) [na][ns][3], branch [na]."""
+    sc = np.zeros(2 * g.na); bo = np.zeros(3 * g.na * g.ns); usex = np.zeros(g.na, np.int32)
+    lib(kind).call("fp_dd_prep", g, sc, bo, usex)
+    return sc, bo, usex
+
+
+def fp_dd(g: Geom, img: np.ndarray, tables, kind="oracle"):
+    """Distance-driven parallel FP (src/fp_par_2d.cc:459-543)."""
+    img = np.ascontiguousarray(img)
+    proj = np.zeros((g.na, g.ns), img.dtype)
+    sc, bo, usex = tables
+    lib(kind).call("fp_dd_" + _suffix(img.dtype), g, img, proj, sc, bo, usex)
+    return proj
+
+
+def fp_dd_grad_prep(g: Geom, kind="oracle"):
+    """Its gradient's tables (src/fp_par_2d.cc:548-599); the restatement's are
+    the ray-driven gradient's, the same arithmetic."""
+    w = np.zeros(g.na); pos = np.zeros(2 * g.na * max(g.nx, g.ny))
+    usex = np.zeros(g.na, np.int32)
+    lib(kind).call("fp_dd_grad_prep" if kind == "ref" else "fp_grad_prep", g, w, pos, usex)
+    return w, pos, usex
+
+
+def fp_dd_grad(g: Geom, proj: np.ndarray, tables, kind="oracle"):
+    """The distance-driven FP's gradient as the reference defines it
+    (src/fp_par_2d.cc:601-655)."""
+    proj = np.ascontiguousarray(proj)
+    img = np.zeros((g.ny, g.nx), proj.dtype)
+    w, pos, usex = tables
+    lib(kind).call("fp_dd_grad_" + _suffix(proj.dtype), g, proj, img, w, pos, usex)
+    return img
+
+
 # ------------------------------------------------- ramp filter / weighting
 def ramp_prep(g: Geom, kind="oracle"):
     h = np.zeros(2 * g.ns + 1)

@@ -221,6 +221,43 @@ int ref_fp_grad_d(const orc_geom* g, const double* proj, double* img, const doub
         .calculate_on_cpu(proj, img, w, pos, usex);
 }
 
+int ref_fp_dd_prep(const orc_geom* g, double* sc, double* bo, int* usex)
+{
+    return ParallelProjection2DDisDrivenPrep(fp_param(g)).calculate_on_cpu(sc, bo, usex);
+}
+
+int ref_fp_dd_f(const orc_geom* g, const float* img, float* proj, const double* sc,
+                const double* bo, const int* usex)
+{
+    return ParallelProjection2DDisDriven<float>(fp_param(g)).calculate_on_cpu(img, proj, sc, bo, usex);
+}
+
+int ref_fp_dd_d(const orc_geom* g, const double* img, double* proj, const double* sc,
+                const double* bo, const int* usex)
+{
+    return ParallelProjection2DDisDriven<double>(fp_param(g))
+        .calculate_on_cpu(img, proj, sc, bo, usex);
+}
+
+int ref_fp_dd_grad_prep(const orc_geom* g, double* weights, double* pos, int* usex)
+{
+    return ParallelProjection2DDisDrivenGradPrep(fp_param(g)).calculate_on_cpu(weights, pos, usex);
+}
+
+int ref_fp_dd_grad_f(const orc_geom* g, const float* proj, float* img, const double* w,
+                     const double* pos, const int* usex)
+{
+    return ParallelProjection2DDisDrivenGrad<float>(fp_param(g))
+        .calculate_on_cpu(proj, img, w, pos, usex);
+}
+
+int ref_fp_dd_grad_d(const orc_geom* g, const double* proj, double* img, const double* w,
+                     const double* pos, const int* usex)
+{
+    return ParallelProjection2DDisDrivenGrad<double>(fp_param(g))
+        .calculate_on_cpu(proj, img, w, pos, usex);
+}
+
 static FilterParam filter_param(const orc_geom* g)
 {
     return FilterParam(g->ns, g->nrow, g->ds, g->dsd, g->type, 0);

@@ -84,6 +84,8 @@ static const ParallelSingleViewBp2DParam kSv(41, 36, 0.8, M_PI / 36, -0.35, 0.1,
                                               0.9, -0.4, 0.25, -1.0);
 static const ParallelProjection2DParam kFp(41, 40, 1.0, M_PI / 40, 0.3, 0.0, 24, 24, 1.0, 1.0,
                                            0.2, -0.1, -1.0);
+static const ParallelProjection2DParam kFpRc(41, 40, 1.0, M_PI / 40, 0.3, 0.0, 24, 24, 1.0, 1.0,
+                                             0.2, -0.1, 11.0);
 static const FilterParam kFilt(33, 5, 0.9, 40.0, 1, 0);
 static const FanWeightingParam kFw(37, 4, 1.3, 0.6, 595.0, 1085.6, 1);
 
@@ -105,6 +107,13 @@ static int cpu()
     std::vector<double> w(kFp.na), pos(2 * kFp.na * kFp.nx);
     CHECK(ParallelProjection2DRayDrivenGradPrep(kFp).calculate_on_cpu(w.data(), pos.data(), u.data()));
     save("fpg_weights", w); save("fpg_pos", pos); save("fpg_usex", u);
+    std::vector<double> d3(3 * kFp.na * kFp.ns);
+    CHECK(ParallelProjection2DDisDrivenPrep(kFp).calculate_on_cpu(s2.data(), d3.data(), u.data()));
+    save("fpdd_sincos", s2); save("fpdd_beginoffset", d3); save("fpdd_usex", u);
+    std::vector<double> rb(2 * kFp.na * kFp.ns);
+    std::vector<int> rn(kFp.na * kFp.ns);
+    CHECK(ParallelProjection2DRayCastingPrepare(kFpRc).calculate_on_cpu(s2.data(), rb.data(), rn.data()));
+    save("fprc_begins", rb); save("fprc_nsteps", rn);
     std::vector<double> h(2 * kFilt.ns + 1);
     CHECK(RampFilterPrep(kFilt).calculate_on_cpu(h.data()));
     save("ramp_taps", h);
@@ -172,6 +181,28 @@ static int gpu_typed(const char* suf, hipStream_t st)
         Dev<T> pr2(load("fp_proj", kFp.na * kFp.ns)), img2(kFp.nx * kFp.ny);
         CHECK(ParallelProjection2DRayDrivenGrad<T>(kFp).calculate_on_gpu(pr2.p, img2.p, w.p, pos.p, ug.p, st));
         save("fpt_" + s, img2.get());
+    }
+    {   // distance-driven FP and the reference's gradient for it
+        Dev<double> sc(2 * kFp.na), bo(3 * kFp.na * kFp.ns);
+        Dev<int> u(kFp.na);
+        CHECK(ParallelProjection2DDisDrivenPrep(kFp).calculate_on_gpu(sc.p, bo.p, u.p, st));
+        Dev<T> img(load("fp_img", kFp.nx * kFp.ny)), proj(kFp.na * kFp.ns);
+        CHECK(ParallelProjection2DDisDriven<T>(kFp).calculate_on_gpu(img.p, proj.p, sc.p, bo.p, u.p, st));
+        save("fpdd_" + s, proj.get());
+        Dev<double> w(kFp.na), pos(2 * kFp.na * kFp.nx);
+        Dev<int> ug(kFp.na);
+        CHECK(ParallelProjection2DDisDrivenGradPrep(kFp).calculate_on_gpu(w.p, pos.p, ug.p, st));
+        Dev<T> pr2(load("fp_proj", kFp.na * kFp.ns)), img2(kFp.nx * kFp.ny);
+        CHECK(ParallelProjection2DDisDrivenGrad<T>(kFp).calculate_on_gpu(pr2.p, img2.p, w.p, pos.p, ug.p, st));
+        save("fpddg_" + s, img2.get());
+    }
+    {   // ray-casting FP (FOV circle of radius 11 inside the 24 x 24 image)
+        Dev<double> sc(2 * kFpRc.na), bg(2 * kFpRc.na * kFpRc.ns);
+        Dev<int> ns(kFpRc.na * kFpRc.ns);
+        CHECK(ParallelProjection2DRayCastingPrepare(kFpRc).calculate_on_gpu(sc.p, bg.p, ns.p, st));
+        Dev<T> img(load("fp_img", kFpRc.nx * kFpRc.ny)), proj(kFpRc.na * kFpRc.ns);
+        CHECK(ParallelProjection2DRayCasting<T>(kFpRc).calculate_on_gpu(img.p, proj.p, sc.p, bg.p, ns.p, st));
+        save("fprc_" + s, proj.get());
     }
     {   // ramp filter, its transpose, fan weighting
         Dev<double> h(2 * kFilt.ns + 1);

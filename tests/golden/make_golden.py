@@ -157,6 +157,31 @@ def fp_fixtures():
              fp_f64=O.fp(g, img.astype(np.float64), t, kind="ref"))
 
 
+# Distance-driven FP: the ray-driven fixture's geometry, and a full circle
+# from a negative start angle (the FP tables wrap the angle, the gradient's
+# tables do not) on a square image whose projections stay on the detector
+# (the reference's gradient reads past the row otherwise); both have rows
+# whose footprint starts left of the first pixel.
+FPDD_CIRCLE = dict(ns=70, na=72, ds=0.7, orbit=2 * math.pi / 72, offset_s=0.25,
+                   orbit_start=-0.3, nx=32, ny=32, dx=1.0, dy=1.0, offset_x=0.0,
+                   offset_y=0.0)
+
+
+def fpdd_fixtures():
+    for name, geo, seed in (("fpdd_small", FP_SMALL, 91), ("fpdd_circle", FPDD_CIRCLE, 93)):
+        g = O.geom(**geo)
+        t = O.fp_dd_prep(g, "ref")
+        tg = O.fp_dd_grad_prep(g, "ref")
+        img = randn(seed, (g.ny, g.nx))
+        proj = randn(seed + 1, (g.na, g.ns))
+        save(name, g, img=img, proj=proj, sincostbl=t[0], beginoffset=t[1], usex=t[2],
+             weights=tg[0], pos=tg[1], usex_grad=tg[2],
+             fp_f32=O.fp_dd(g, img, t, kind="ref"),
+             fp_f64=O.fp_dd(g, img.astype(np.float64), t, kind="ref"),
+             grad_f32=O.fp_dd_grad(g, proj, tg, kind="ref"),
+             grad_f64=O.fp_dd_grad(g, proj.astype(np.float64), tg, kind="ref"))
+
+
 def filter_fixtures():
     out = {}
     for typ in (0, 1, 2):
@@ -191,4 +216,5 @@ if __name__ == "__main__":
     svbp_fixtures()
     rc_fixtures()
     fp_fixtures()
+    fpdd_fixtures()
     filter_fixtures()

@@ -52,6 +52,12 @@ def test_class_api_prep_tables_cpu(driver, tmp_path):
         np.testing.assert_array_equal(_read(tmp_path, name), a)
     for a, name in zip(O.fp_grad_prep(O.geom(**FP)), ("fpg_weights", "fpg_pos", "fpg_usex")):
         np.testing.assert_array_equal(_read(tmp_path, name), a)
+    for a, name in zip(O.fp_dd_prep(O.geom(**FP)), ("fpdd_sincos", "fpdd_beginoffset",
+                                                    "fpdd_usex")):
+        np.testing.assert_array_equal(_read(tmp_path, name), a)
+    trc = O.fp_rc_prep(O.geom(**FP, fov=11.0))
+    np.testing.assert_array_equal(_read(tmp_path, "fprc_begins"), trc[1])
+    np.testing.assert_array_equal(_read(tmp_path, "fprc_nsteps"), trc[2])
     np.testing.assert_array_equal(_read(tmp_path, "ramp_taps"), O.ramp_prep(O.geom(**FILT)))
 
 
@@ -93,14 +99,25 @@ def test_class_api_operators_gpu(driver, tmp_path):
             "fw": O.fanw(gw, inputs["fw_in"].astype(dt)),
             "fwg": O.fanw(gw, inputs["fw_in"].astype(dt), grad=True),
         }
-        # single-view BP: bit-identical to the reference's arithmetic
+        # single-view BP, ray-casting FP and the distance-driven pair:
+        # bit-identical to the reference's arithmetic
         np.testing.assert_array_equal(
             _read(tmp_path, f"svbp_{suf}"),
             O.par_svbp(gp, inputs["par_proj"].astype(dt), tp).ravel().astype(np.float64))
+        grc = O.geom(**FP, fov=11.0)
+        np.testing.assert_array_equal(
+            _read(tmp_path, f"fprc_{suf}"),
+            O.fp_rc(grc, inputs["fp_img"].astype(dt), O.fp_rc_prep(grc)).ravel().astype(np.float64))
+        np.testing.assert_array_equal(
+            _read(tmp_path, f"fpdd_{suf}"),
+            O.fp_dd(gq, inputs["fp_img"].astype(dt), O.fp_dd_prep(gq)).ravel().astype(np.float64))
+        np.testing.assert_array_equal(
+            _read(tmp_path, f"fpddg_{suf}"),
+            O.fp_dd_grad(gq, inputs["fp_proj"].astype(dt),
+                         O.fp_dd_grad_prep(gq)).ravel().astype(np.float64))
         for name, want in ref.items():
             got = _read(tmp_path, f"{name}_{suf}")
-            # the float transposes scatter with atomics; the FP64 ones too
-            t = 1e-5 if (suf == "f32" or name in ("fan_bpt", "par_bpt")) else tol
+            t = 1e-5 if suf == "f32" else tol
             assert rel_l2(got, want.ravel()) < t, (name, suf, rel_l2(got, want.ravel()))
 
 

@@ -146,6 +146,33 @@ def test_fanw_small(typ):
     np.testing.assert_array_equal(O.fanw(g, x, grad=True), d[f"grad_f32_{typ}"])
 
 
+@pytest.mark.parametrize("name", ["fpdd_small", "fpdd_circle"])
+def test_fp_dd(name):
+    """Distance-driven FP and its gradient: tables and outputs bitwise.  Both
+    geometries have rows whose footprint starts left of the first pixel (the
+    reference's unsigned index skips them)."""
+    geo, d = load_golden(name)
+    g = G(geo)
+    t = O.fp_dd_prep(g)
+    for a, k in zip(t, ("sincostbl", "beginoffset", "usex")):
+        np.testing.assert_array_equal(a, d[k])
+    tg = O.fp_dd_grad_prep(g)
+    np.testing.assert_array_equal(tg[0], d["weights"])
+    np.testing.assert_array_equal(tg[1][:d["pos"].size], d["pos"])
+    np.testing.assert_array_equal(tg[2], d["usex_grad"])
+    np.testing.assert_array_equal(O.fp_dd(g, d["img"], t), d["fp_f32"])
+    np.testing.assert_array_equal(O.fp_dd(g, d["img"].astype(np.float64), t), d["fp_f64"])
+    np.testing.assert_array_equal(O.fp_dd_grad(g, d["proj"], tg), d["grad_f32"])
+    np.testing.assert_array_equal(O.fp_dd_grad(g, d["proj"].astype(np.float64), tg),
+                                  d["grad_f64"])
+    bo = d["beginoffset"].reshape(g.na, g.ns, 3)
+    left = np.minimum(bo[..., 0] - bo[..., 2], bo[..., 0] + bo[..., 2])
+    off = bo[..., 1]
+    n_rows = np.where(d["usex"][:, None] != 0, g.ny, g.nx)
+    # some footprint leaves the image on the left within the scan's rows
+    assert np.any((left < 0) | (left + off * (n_rows - 1) < 0))
+
+
 @pytest.mark.skipif(not O.have_ref(), reason="compiled reference not built here")
 def test_restatement_matches_compiled_reference_fresh_seeds():
     """Beyond the stored vectors: fresh seeds, every operator, bitwise."""
@@ -165,3 +192,11 @@ def test_restatement_matches_compiled_reference_fresh_seeds():
     t = O.fp_prep(g)
     im = randn(4000, (g.ny, g.nx))
     np.testing.assert_array_equal(O.fp(g, im, t), O.fp(g, im, t, kind="ref"))
+    from tests.golden.make_golden import FPDD_CIRCLE
+    for geo in (FP_SMALL, FPDD_CIRCLE):
+        g = O.geom(**geo)
+        t, tg = O.fp_dd_prep(g), O.fp_dd_grad_prep(g)
+        im = randn(5000, (g.ny, g.nx))
+        np.testing.assert_array_equal(O.fp_dd(g, im, t), O.fp_dd(g, im, t, kind="ref"))
+        p = randn(5001, (g.na, g.ns))
+        np.testing.assert_array_equal(O.fp_dd_grad(g, p, tg), O.fp_dd_grad(g, p, tg, kind="ref"))

@@ -122,6 +122,41 @@ def test_raycasting_fp_of_constant_is_chord_length():
     assert np.all(proj[:, chord == 0] == 0)
 
 
+def test_fp_dd_of_constant_image():
+    """Distance-driven FP of a constant c at 0 and 90 degrees: every row
+    (column) the footprint overlaps averages to c, so a channel is c times
+    the image's height ny dy (width nx dx) -- and 0 where the footprint
+    starts left of the first pixel, the reference's unsigned-index rule."""
+    c = 1.75
+    g = O.geom(ns=50, na=2, ds=0.9, orbit=math.pi / 2, offset_s=0.35, orbit_start=0.0,
+               nx=30, ny=20, dx=1.1, dy=0.8, offset_x=0.2, offset_y=-0.3)
+    sc, bo, usex = O.fp_dd_prep(g)
+    proj = O.fp_dd(g, np.full((g.ny, g.nx), c), (sc, bo, usex))
+    assert list(usex) == [1, 0]
+    b = bo.reshape(g.na, g.ns, 3)
+    left = np.minimum(b[..., 0] - b[..., 2], b[..., 0] + b[..., 2])
+    n_cells = np.array([g.nx, g.ny])[:, None]
+    expect = np.where((left >= 0) & (np.floor(left) < n_cells - 1),
+                      c * np.array([g.ny * g.dy, g.nx * g.dx])[:, None], 0.0)
+    assert (expect > 0).sum() >= 20 and (expect == 0).sum() >= 10
+    np.testing.assert_allclose(proj, expect, rtol=1e-12, atol=1e-12)
+
+
+def test_fp_dd_grad_of_constant_sinogram():
+    """Distance-driven gradient of a constant sinogram c: an interior pixel's
+    span [mid - o1, mid + o1] overlaps channel cells of total length 2|o1|,
+    each adding overlap c / 2, times the view weight, and |o1| w = dx dy / ds
+    in both branches: the pixel is c na dx dy / ds."""
+    c = 0.6
+    g = O.geom(ns=90, na=37, ds=0.8, orbit=math.pi / 37, offset_s=0.1, orbit_start=0.05,
+               nx=24, ny=24, dx=1.2, dy=1.2)
+    img = O.fp_dd_grad(g, np.full((g.na, g.ns), c), O.fp_dd_grad_prep(g))
+    inner = img[1:-1, 1:-1]
+    np.testing.assert_allclose(inner, c * g.na * g.dx * g.dy / g.ds, rtol=1e-12)
+    # border pixels are one-sided along the branch's axis: strictly less
+    assert np.all(img[0, :] < inner.max()) and np.all(img[:, 0] < inner.max())
+
+
 def test_fan_bp_constant_sinogram_matches_source_distances():
     """BP of a constant c is c*scale*sum_beta dsd^2/|P - S(beta)|^2, with the
     source at S = (-dso sin b, dso cos b) computed here independently."""
