@@ -507,7 +507,9 @@ def bench_fbp_backward(torch, F, proj, reps):
     a1 = F.fan_backprojection_2d_grad(gimg, **kwb, deterministic=True)
     a2 = F.fan_backprojection_2d_grad(gimg, **kwb, deterministic=True)
     pv = B * n * n * na
-    groups = math.ceil(B / 8)  # the scatter kernel carries 8 slices per workgroup
+    # the transpose's algorithmic work is the forward's interpolation, term by
+    # term: geometry per pixel*view once per 16-slice group, 2 per slice
+    groups = math.ceil(B / 16)
     slots = n * n * na * FAN_GEOM_SLOTS * groups + pv * FAN_SLICE_SLOTS
     return {
         "workload": "16 x 512^2 fan FBP (720 views, 736 ch): forward + backward via autograd; "
@@ -516,7 +518,13 @@ def bench_fbp_backward(torch, F, proj, reps):
         "bpt_ms": round(ms_bpt, 4), "bpt_det_ms": round(ms_bpt_det, 4),
         "bpt_gpixel_views_per_s": round(pv / (ms_bpt * 1e-3) / 1e9, 1),
         "bpt_det_gpixel_views_per_s": round(pv / (ms_bpt_det * 1e-3) / 1e9, 1),
-        "bpt_valu_slot_model_frac": round(slots / (ms_bpt * 1e-3) / PEAK_VALU_SLOTS, 4),
+        "roofline_bpt": {"kernel": "bp_gather_t_kernel<FanGeo<false>, 16, 4, 128>",
+                         "bound": "valu",
+                         "achieved": round(slots / (ms_bpt * 1e-3) / 1e12, 3),
+                         "peak": round(PEAK_VALU_SLOTS / 1e12, 3),
+                         "unit": "Tslot/s (FP32 VALU lane-issues)",
+                         "frac": round(slots / (ms_bpt * 1e-3) / PEAK_VALU_SLOTS, 4),
+                         "slots_per_launch": slots},
         "adjoint_gap": gap,
         "det_bitwise_repeatable": bool(torch.equal(a1, a2)),
     }

@@ -125,7 +125,10 @@ struct GatherShape {
     static constexpr bool kQuad = NB % 4 == 0;
     static constexpr int kGroups = kQuad ? NB / 4 : 1;
     static constexpr int kTiles = WMAX / kGatherTile;  // tiles along w per chunk
-    static constexpr int kImgFloats = R * WMAX * NB;
+    // one float4 of padding per staged row: lanes of a wave on different
+    // rows then fall on different LDS banks
+    static constexpr int kRowF4 = (kQuad ? kGroups * WMAX : WMAX / 4) + 1;
+    static constexpr int kImgFloats = R * kRowF4 * 4;
     static constexpr int kAnchorDwords =
         kGatherViews * kTiles * static_cast<int>(sizeof(typename Geo::Anchor) / 4);
     static constexpr size_t kLds = kImgFloats * sizeof(float) + kAnchorDwords * 4 + 64;
@@ -256,41 +259,46 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
 
         for (int cw = W0 & ~(kGatherTile - 1); cw <= W1; cw += WMAX) {
             if (cw != (W0 & ~(kGatherTile - 1))) __syncthreads();  // previous chunk done
-            // ---- stage the image: rows u0 .. u0+rmax-1, pixels cw .. cw+WMAX-1
+            // ---- stage the image: rows u0 .. u0+rmax-1, pixels cw .. cw+WMAX-1.
+            // A wave stages whole rows: one buffer descriptor per (row, slice
+            // group), then K DMAs that differ only in the pixel block.
             if constexpr (S::kQuad) {
-                constexpr int JOBS = R * NG * K;
                 const int lane_px = lane >> 2;
                 const unsigned int lane_off = (lane & 3) * slice_bytes;
-                for (int j = wave; j < JOBS; j += V) {
-                    const int r = j / (NG * K), g = (j / K) % NG, k = j % K;
-                    const int ng = nb - 4 * g;
-                    if (r >= rmax || ng <= 0) continue;
-                    const float* base = src + (static_cast<size_t>(b0 + 4 * g) * nu + u0 + r) * nw;
-                    const unsigned int range = ng >= 4 ? 4u * slice_bytes : ng * slice_bytes;
-                    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                        const_cast<float*>(base), 0, range, 0x00020000);
-                    const int px = cw + 16 * k + lane_px;
-                    const unsigned int voff = static_cast<unsigned int>(px) < static_cast<unsigned int>(nw)
-                                                  ? lane_off + static_cast<unsigned int>(px) * 4u
-                                                  : 0x80000000u;
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                        rs,
-                        (__attribute__((address_space(3))) void*)(simg +
-                                                                  ((r * NG + g) * WMAX + 16 * k) * 4),
-                        4, voff, 0, 0, 0);
+                for (int r = wave; r < rmax; r += V) {
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        const int ng = nb - 4 * g;
+                        if (ng <= 0) break;
+                        const float* base =
+                            src + (static_cast<size_t>(b0 + 4 * g) * nu + u0 + r) * nw;
+                        const unsigned int range = ng >= 4 ? 4u * slice_bytes : ng * slice_bytes;
+                        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                            const_cast<float*>(base), 0, range, 0x00020000);
+                        float* dst = simg + (r * S::kRowF4 + g * WMAX) * 4;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                            const int px = cw + 16 * k + lane_px;
+                            const unsigned int voff =
+                                static_cast<unsigned int>(px) < static_cast<unsigned int>(nw)
+                                    ? lane_off + static_cast<unsigned int>(px) * 4u
+                                    : 0x80000000u;
+                            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                                rs, (__attribute__((address_space(3))) void*)(dst + 64 * k), 4,
+                                voff, 0, 0, 0);
+                        }
+                    }
                 }
             } else {
-                constexpr int JOBS = R * (WMAX / 64);
-                for (int j = wave; j < JOBS; j += V) {
-                    const int r = j / (WMAX / 64), k = j % (WMAX / 64);
-                    if (r >= rmax || nb <= 0) continue;
+                for (int r = wave; r < rmax; r += V) {
                     const float* base = src + (static_cast<size_t>(b0) * nu + u0 + r) * nw;
                     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                         const_cast<float*>(base), 0, static_cast<unsigned int>(nw) * 4u, 0x00020000);
-                    const int px = cw + 64 * k + lane;
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                        rs, (__attribute__((address_space(3))) void*)(simg + r * WMAX + 64 * k), 4,
-                        static_cast<unsigned int>(px) * 4u, 0, 0, 0);
+#pragma unroll
+                    for (int k = 0; k < WMAX / 64; ++k)
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            rs, (__attribute__((address_space(3))) void*)(simg + r * S::kRowF4 * 4 + 64 * k),
+                            4, static_cast<unsigned int>(cw + 64 * k + lane) * 4u, 0, 0, 0);
                 }
             }
             // ---- the anchors of the chunk's tiles, for the workgroup's views
@@ -355,24 +363,27 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
                 float p1 = 0.0f;
                 if constexpr (S::kQuad) {
                     const float4* q =
-                        reinterpret_cast<const float4*>(simg) + (r * NG) * WMAX + (w - cw);
+                        reinterpret_cast<const float4*>(simg) + r * S::kRowF4 + (w - cw);
 #pragma unroll
                     for (int g = 0; g < NG; ++g) pv[g] = q[g * WMAX];
                 } else {
-                    p1 = simg[r * WMAX + (w - cw)];
+                    p1 = simg[r * S::kRowF4 * 4 + (w - cw)];
                 }
                 const int u = u0 + r;
                 const int ix = frame ? u : w, iy = frame ? w : u;
-                const Anchor A = anch[wave * K + ((w - cw) >> 4)];
+                const Anchor A = *reinterpret_cast<const Anchor*>(
+                    reinterpret_cast<const unsigned char*>(anch) +
+                    static_cast<unsigned int>(wave * K + ((w - cw) >> 4)) *
+                        static_cast<unsigned int>(sizeof(Anchor)));
                 const int ic = min((ix & ~(kGatherTile - 1)) + kGatherTile / 2, P.nx - 1);
                 const int jc = min((iy & ~(kGatherTile - 1)) + kGatherTile / 2, P.ny - 1);
                 float f, wt;
                 Geo::pixel(Pg, A, static_cast<float>(ix - ic), static_cast<float>(iy - jc), f, wt);
                 const float fr = __builtin_amdgcn_fractf(f);
                 const int s = A.is0 + static_cast<int>(f);
-                // the forward's edge rule, evaluated without a branch
-                const bool on = (s >= 0) & ((s < P.ns - 1) | ((s == P.ns - 1) & (fr == 0.0f)));
-                wt = ((A.inside != 0) | on) ? wt : 0.0f;
+                // the forward's edge rule, only for tiles that reach past the
+                // detector (few, and the same for most lanes of a wave)
+                if (!A.inside) wt = on_detector(s, fr, P.ns) ? wt : 0.0f;
                 const float w1 = wt * fr, w0 = wt - w1;
                 const float wc = s == c ? w0 : (s + 1 == c ? w1 : 0.0f);
                 if constexpr (S::kQuad) {
