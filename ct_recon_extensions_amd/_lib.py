@@ -81,6 +81,7 @@ _SIGS = {
     "ctr_fp": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
     "ctr_fp_rc": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, C.c_double, _P, _P, _P, _P]),
     "ctr_fp_t": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_uint, _P]),
+    "ctr_fp_t_noalloc": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_uint, _P]),
     "ctr_fp_dd": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
     "ctr_fp_dd_grad": (C.c_int, [_G, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_uint, _P]),
     "ctr_ramp": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, _P]),

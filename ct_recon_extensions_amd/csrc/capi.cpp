@@ -39,6 +39,9 @@ int fp_dd(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
 int fp_dd_grad(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
                const double* w, const double* pos, const int* usex, unsigned int pos_stride,
                hipStream_t st);
+int fp_par_t_noalloc(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
+                     const double* w, const double* pos, const int* usex,
+                     unsigned int pos_stride, hipStream_t st);
 int ramp(const ctr_geom& g, int dtype, const void* in, void* out, unsigned int nrow,
          const double* filter, int weight_type, hipStream_t st);
 int fan_weighting(const ctr_geom& g, int dtype, const void* in, void* out, unsigned int nrow,
@@ -260,6 +263,22 @@ int ctr_fp_dd_grad(const ctr_geom* g, int dtype, const void* proj, void* img, in
     return guarded("ctr_fp_dd_grad", [&] {
         return ctr::fp_dd_grad(*g, dtype, proj, img, nbatch, weights, pos, usex, pos_stride,
                                static_cast<hipStream_t>(stream));
+    });
+}
+
+int ctr_fp_t_noalloc(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                     const double* weights, const double* pos, const int* usex,
+                     unsigned int pos_stride, void* stream)
+{
+    int rc = check_common(g, dtype, proj, img, nbatch, "ctr_fp_t_noalloc");
+    if (rc) return rc;
+    CTR_REQUIRE(weights && pos && usex, CTR_ERR_ARG, "ctr_fp_t_noalloc: null table");
+    const unsigned int need = g->nx > g->ny ? g->nx : g->ny;
+    CTR_REQUIRE(pos_stride >= need, CTR_ERR_ARG,
+                "ctr_fp_t_noalloc: pos_stride must be >= max(nx, ny)");
+    return guarded("ctr_fp_t_noalloc", [&] {
+        return ctr::fp_par_t_noalloc(*g, dtype, proj, img, nbatch, weights, pos, usex,
+                                     pos_stride, static_cast<hipStream_t>(stream));
     });
 }
 

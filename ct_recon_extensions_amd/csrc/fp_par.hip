@@ -31,7 +31,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <type_traits>
+#include <unordered_map>
 
 namespace ctr {
 
@@ -516,15 +519,25 @@ int launch_fpt(const FpProblem& P, const T* proj, T* img, const double* w, const
     return check_launch("fpt_kernel");
 }
 
+// Bytes of FP64 partials the border pass needs (nchunks x nbatch x nborder).
+size_t fpt_border_bytes(int nx, int ny, int na, int nbatch)
+{
+    const size_t nborder = 2 * static_cast<size_t>(nx) + 2 * (static_cast<size_t>(ny) - 2);
+    return static_cast<size_t>(div_up(na, kBorderViews)) * nbatch * nborder * sizeof(double);
+}
+
+// ws: caller workspace of at least fpt_border_bytes(...) bytes, or null for a
+// stream-ordered allocation.  Every partial is written before it is read.
 template <int NB>
 int launch_fpt_border(const FpProblem& P, const float* proj, float* img, const double* w,
-                      const double* pos, const int* usex, int stride, hipStream_t st)
+                      const double* pos, const int* usex, int stride, void* ws, hipStream_t st)
 {
     const int nborder = 2 * P.nx + 2 * (P.ny - 2);
     const int nchunks = div_up(P.na, kBorderViews);
-    double* partial = nullptr;
-    CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial),
-                           static_cast<size_t>(nchunks) * P.nbatch * nborder * sizeof(double), st));
+    double* partial = static_cast<double*>(ws);
+    if (!ws)
+        CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial),
+                               fpt_border_bytes(P.nx, P.ny, P.na, P.nbatch), st));
     hipLaunchKernelGGL((fpt_border_partial_kernel<NB>),
                        dim3(div_up(nborder, 256), nchunks, div_up(P.nbatch, NB)), dim3(256), 0, st,
                        proj, partial, w, pos, usex, P, stride, nborder);
@@ -534,7 +547,7 @@ int launch_fpt_border(const FpProblem& P, const float* proj, float* img, const d
                            dim3(256), 0, st, partial, img, P, nborder, nchunks);
         rc = check_launch("fpt_border_reduce_kernel");
     }
-    CTR_HIP(hipFreeAsync(partial, st));
+    if (!ws) CTR_HIP(hipFreeAsync(partial, st));
     return rc;
 }
 
@@ -579,7 +592,7 @@ bool fpt_engine_applies(const ctr_geom& g)
 
 template <typename T>
 int fpt_typed(const ctr_geom& g, const T* proj, T* img, int nbatch, const double* w,
-              const double* pos, const int* usex, unsigned int stride, hipStream_t st)
+              const double* pos, const int* usex, unsigned int stride, void* ws, hipStream_t st)
 {
     if constexpr (std::is_same<T, float>::value) {
         if (fpt_engine_applies(g) && g.nx >= 3 && g.ny >= 3) {
@@ -597,8 +610,8 @@ int fpt_typed(const ctr_geom& g, const T* proj, T* img, int nbatch, const double
             if (rc) return rc;
             FpProblem P{static_cast<int>(g.ns), static_cast<int>(g.na), static_cast<int>(g.nx),
                         static_cast<int>(g.ny), nbatch, g.dx, g.dy};
-            return nbatch >= 8 ? launch_fpt_border<8>(P, proj, img, w, pos, usex, s, st)
-                               : launch_fpt_border<1>(P, proj, img, w, pos, usex, s, st);
+            return nbatch >= 8 ? launch_fpt_border<8>(P, proj, img, w, pos, usex, s, ws, st)
+                               : launch_fpt_border<1>(P, proj, img, w, pos, usex, s, ws, st);
         }
     }
     FpProblem P{static_cast<int>(g.ns), static_cast<int>(g.na), static_cast<int>(g.nx),
@@ -635,9 +648,66 @@ int fp_par_t(const ctr_geom& g, int dtype, const void* proj, void* img, int nbat
                 "ray-driven FP transpose: sizes must be positive");
     if (dtype == CTR_F32)
         return fpt_typed(g, static_cast<const float*>(proj), static_cast<float*>(img), nbatch, w,
-                         pos, usex, pos_stride, st);
+                         pos, usex, pos_stride, nullptr, st);
     return fpt_typed(g, static_cast<const double*>(proj), static_cast<double*>(img), nbatch, w,
-                     pos, usex, pos_stride, st);
+                     pos, usex, pos_stride, nullptr, st);
+}
+
+// ---------------------------------------------- allocation-free FP transpose
+// The only device memory the FP transpose uses beyond its inputs and output
+// is the border pass's FP64 partials (float32 buffers, geometries on the BP
+// engine).  ctr_fp_t_noalloc keeps them in this file-scope array instead of a
+// stream-ordered allocation: kFptScratchBytes per device, held until the
+// process ends.  Kernels take its device address, which hipGetSymbolAddress
+// returns once per device (the name itself is a host-side stand-in).
+constexpr size_t kFptScratchBytes = size_t{16} << 20;
+static __device__ __align__(256) unsigned char scratch[kFptScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::unordered_map<int, void*> g_scratch_ptr;  // device -> address of `scratch` there
+
+int scratch_address(void** out)
+{
+    int dev = 0;
+    CTR_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void* p = nullptr;
+        CTR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        it = g_scratch_ptr.emplace(dev, p).first;
+    }
+    *out = it->second;
+    return CTR_OK;
+}
+}  // namespace
+
+int fp_par_t_noalloc(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
+                     const double* w, const double* pos, const int* usex,
+                     unsigned int pos_stride, hipStream_t st)
+{
+    CTR_REQUIRE(g.ns >= 1 && g.na >= 1 && g.nx >= 1 && g.ny >= 1, CTR_ERR_ARG,
+                "ray-driven FP transpose: sizes must be positive");
+    if (dtype != CTR_F32)  // the float64 transpose uses no scratch
+        return fpt_typed(g, static_cast<const double*>(proj), static_cast<double*>(img), nbatch,
+                         w, pos, usex, pos_stride, nullptr, st);
+    // only the engine path (with its border pass) uses the scratch
+    const bool border = fpt_engine_applies(g) && g.nx >= 3 && g.ny >= 3;
+    const size_t need = border ? fpt_border_bytes(static_cast<int>(g.nx), static_cast<int>(g.ny),
+                                                  static_cast<int>(g.na), nbatch)
+                               : 0;
+    if (need > kFptScratchBytes) {
+        set_error("ctr_fp_t_noalloc: the border partials need " + std::to_string(need) +
+                  " bytes, more than the " + std::to_string(kFptScratchBytes) +
+                  "-byte scratch (use ctr_fp_t, or fewer slices per call)");
+        return CTR_ERR_SHAPE;
+    }
+    void* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc) return rc;
+    return fpt_typed(g, static_cast<const float*>(proj), static_cast<float*>(img), nbatch, w, pos,
+                     usex, pos_stride, ws, st);
 }
 
 }  // namespace ctr

@@ -198,6 +198,19 @@ int ctr_fp_dd(const ctr_geom* g, int dtype, const void* img, void* proj, int nba
 int ctr_fp_dd_grad(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
                    const double* weights, const double* pos, const int* usex,
                    unsigned int pos_stride, void* stream);
+/* ctr_fp_t without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory during the call (code
+ * under HIP graph capture, serving loops, harnesses that forbid it).  Same
+ * arguments and results as ctr_fp_t; the only memory the transpose needs
+ * beyond its buffers (the border pass's FP64 partial sums, float32 on the
+ * usual geometries) lives in a static 16 MiB device array per GPU, held
+ * until the process ends, so nothing is allocated or freed.  A call needing
+ * more (na/16 x nbatch x (2nx + 2ny - 4) doubles) returns CTR_ERR_SHAPE.
+ * Calls that overlap on one GPU share that array and race: issue them on
+ * one stream, or order them. */
+int ctr_fp_t_noalloc(const ctr_geom* g, int dtype, const void* proj, void* img, int nbatch,
+                     const double* weights, const double* pos, const int* usex,
+                     unsigned int pos_stride, void* stream);
 /* Ramp filter, RampFilter<T> / RampFilterGrad<T> (src/filter.cc:67-124):
  * out[r][is] = ds * sum_k in[r][k] h[is-k] over nrow_total rows of ns.
  * filter: device copy of the 2ns+1 FP64 taps.  Optional fused fan weighting
