@@ -175,6 +175,13 @@ def profile_kernels(torch, F, which):
         for _ in range(3):
             rb(graph=False)
         img = rb.img_flat
+    elif which == "helical":
+        from ct_recon_extensions_amd.geometry import HelicalInterpolationParam
+        p = HelicalInterpolationParam(**helical_scan(HELICAL_NZ, HELICAL_NA))
+        x = torch.randn((p.na, p.nt, p.ns), generator=g, device="cuda")
+        for mode in (360, 180):
+            for _ in range(3):
+                img = F.helical_interpolation(x, p, mode, check=False)
     elif which == "filter5":
         from ct_recon_extensions_amd import phantom as P
         x, kw, _ = P.shepp_logan_fan_batch(32, PRECISION_SEED, "cuda")
@@ -530,6 +537,80 @@ def bench_fbp_backward(torch, F, proj, reps):
     }
 
 
+HELICAL_SEED = 20261208
+
+
+def helical_scan(nz, na):
+    """Helical workload: 736 channels of 1 mm and 32 rows at SID 595 / SDD
+    1085.6 mm (17.5 mm collimation at the isocentre), 1152 views per
+    rotation, pitch 1, nz slices of 1 mm thickness at 1 mm spacing starting
+    1160 views into the scan."""
+    m = 32 * 595.0 / 1085.6 / 1152
+    return dict(ns=736, nt=32, na=na, ds=1.0, dt=1.0, offset_s=1.25, offset_t=0.0,
+                orbit_start=0.0, orbit=2 * math.pi / 1152, couch_begin=0.0, couch_mov=m,
+                view_per_rot=1152, nz=nz, dz=1.0, z0=1160 * m, thickness=1.0, dso=595.0,
+                dsd=1085.6)
+
+
+HELICAL_NZ, HELICAL_NA = 96, 8640
+
+
+def bench_helical(torch, F, reps, with_cpu=True):
+    """Helical rebinning (src/helical_interpolation.cc:81-269), 360LI and
+    180LI, of one seeded N(0,1) scan of 8640 views x 32 rows x 736 channels
+    (814 MB, float32) into 96 fan sinograms of 1152 views: GPU time per
+    launch against the HBM roofline, whose compulsory bytes are the views
+    the slices' windows span (every row of each) plus the output; and the
+    reference CPU path on one thread for 2 of the slices."""
+    from ct_recon_extensions_amd.geometry import HelicalInterpolationParam
+    geo = helical_scan(HELICAL_NZ, HELICAL_NA)
+    p = HelicalInterpolationParam(**geo)
+    g = torch.Generator(device="cuda").manual_seed(HELICAL_SEED)
+    x = torch.randn((p.na, p.nt, p.ns), generator=g, device="cuda")
+    res = {"workload": f"helical scan {p.na} views x {p.nt} rows x {p.ns} ch (1152 views/rot, "
+                       f"pitch 1, 17.5 mm collimation) -> {p.nz} slices of 1 mm; seeded N(0,1)",
+           "seed": HELICAL_SEED}
+    m = geo["couch_mov"]
+    cv = [round((geo["z0"] + k * geo["dz"] - geo["couch_begin"]) / m) for k in range(p.nz)]
+    out_bytes = p.nz * p.view_per_rot * p.ns * 4
+    samples = p.nz * p.view_per_rot * p.ns
+    for mode in (360, 180):
+        out = F.helical_interpolation(x, p, mode)
+        torch.cuda.synchronize()
+        ms = time_events(lambda: F.helical_interpolation(x, p, mode, check=False), reps, torch)
+        if mode == 360:
+            v0, v1 = cv[0] - p.view_per_rot, cv[-1] + p.view_per_rot - 1
+        else:
+            v0 = cv[0] - p.view_per_rot // 2 - 1
+            v1 = cv[-1] - p.view_per_rot // 2 + p.view_per_rot - 1
+        in_bytes = (v1 - v0 + 1) * p.nt * p.ns * 4
+        bytes_ = in_bytes + out_bytes
+        gbs = bytes_ / (ms * 1e-3) / 1e9
+        r = {"gpu_ms": round(ms, 4), "msamples_per_s": round(samples / (ms * 1e-3) / 1e6, 1),
+             "roofline": {"kernel": f"helical_kernel<float, {mode}>", "bound": "hbm",
+                          "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(gbs / PEAK_HBM_GBS, 4),
+                          "compulsory_bytes": bytes_},
+             "nan_free": bool(torch.isfinite(out).all())}
+        if with_cpu:
+            from oracle import oracle as O
+            kind, label = _cpu_kind()
+            gs = dict(geo, nz=2)
+            hs = O.heli(**gs)
+            host = x.cpu().numpy()
+            t0 = time.perf_counter()
+            ref, code = O.helical(hs, host, mode, kind=kind)
+            sec = time.perf_counter() - t0
+            r["cpu_baseline"] = {
+                "value": round(2 * p.view_per_rot * p.ns / sec / 1e6, 2), "unit": "Msample/s",
+                "cores": 1, "kind": label,
+                "sample": f"the first 2 of the {p.nz} slices on one thread, {sec:.2f}s"}
+            r["bit_identical_first_2_slices"] = bool(
+                code == 1 and np.array_equal(out[:2].cpu().numpy(), ref))
+        res[f"li{mode}"] = r
+    return res
+
+
 def _cpu_kind():
     from oracle import oracle as O
     return ("ref", "reference") if O.have_ref() else ("oracle", "port")
@@ -685,13 +766,14 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 256^2 parallel FBP")
     ap.add_argument("--no-svbp", action="store_true", help="skip the single-view parallel BP")
     ap.add_argument("--no-rc", action="store_true", help="skip the ray-casting FP")
+    ap.add_argument("--no-helical", action="store_true", help="skip the helical rebinning")
     ap.add_argument("--no-bwd", action="store_true",
                     help="skip the differentiable FBP's backward (BP transpose)")
     ap.add_argument("--profile-bp", action="store_true",
                     help="counter-collection mode: three fan BP launches of the workload, then a "
                          "256 MiB device copy from a cold source (calibrates FETCH/WRITE_SIZE)")
     ap.add_argument("--profile", choices=("bp", "filter", "lsq", "svbp", "rc", "bpt", "ragged",
-                                          "filter5"), default=None,
+                                          "filter5", "helical"), default=None,
                     help="counter-collection mode for one kernel family: 'bp' as --profile-bp; "
                          "'filter' three fused fan-weighted ramp filters (default implementation) of the "
                          "workload; 'lsq' three FP and FP^T launches of the least-squares "
@@ -700,8 +782,9 @@ def main():
                          "transposes of the workload in each mode (scatter, then "
                          "deterministic); 'ragged' three eager runs of the ragged batch "
                          "(16x16 tiles, 32 views per round); 'filter5' three BF16x3 filters "
-                         "of the 32 x 1440 x 1472 precision workload; each followed by the "
-                         "calibration copy")
+                         "of the 32 x 1440 x 1472 precision workload; 'helical' three 360LI "
+                         "and three 180LI rebinnings of the helical workload; each followed "
+                         "by the calibration copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path shards by slice with no exchange, so "
@@ -898,6 +981,11 @@ def main():
         rc_ = result["raycasting_fp"]
         print(f"[bench] ray-casting FP {rc_['gpu_ms']:.3f} ms, {rc_['gsamples_per_s']:.0f} "
               f"Gsamples/s", file=sys.stderr, flush=True)
+    if not args.no_helical:
+        result["helical_rebinning"] = bench_helical(torch, F, 20, with_cpu=not args.no_cpu)
+        hb = result["helical_rebinning"]
+        print(f"[bench] helical 360LI {hb['li360']['gpu_ms']:.3f} ms, 180LI "
+              f"{hb['li180']['gpu_ms']:.3f} ms", file=sys.stderr, flush=True)
     if not args.no_precision:
         result["filter_precision"] = bench_filter_precision(torch, args.steps)
         fp_ = result["filter_precision"]

@@ -47,6 +47,20 @@ class CtrFanBpLaunch(C.Structure):
     _fields_ = [("tile", C.c_int), ("vchunk", C.c_int), ("nb", C.c_int)]
 
 
+class CtrHelicalGeom(C.Structure):
+    """Mirror of ``ctr_helical_geom`` (include/ctr_api.h)."""
+
+    _fields_ = [
+        ("ns", C.c_uint), ("nt", C.c_uint), ("na", C.c_uint),
+        ("ds", C.c_double), ("dt", C.c_double), ("offset_s", C.c_double),
+        ("offset_t", C.c_double), ("orbit_start", C.c_double), ("orbit", C.c_double),
+        ("couch_begin", C.c_double), ("couch_mov", C.c_double),
+        ("view_per_rot", C.c_uint), ("nz", C.c_uint),
+        ("dz", C.c_double), ("z0", C.c_double), ("thickness", C.c_double),
+        ("dso", C.c_double), ("dsd", C.c_double),
+    ]
+
+
 class CtrError(RuntimeError):
     pass
 
@@ -55,6 +69,7 @@ _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
 _G = C.POINTER(CtrGeom)
+_H = C.POINTER(CtrHelicalGeom)
 
 _SIGS = {
     "ctr_last_error": (C.c_char_p, []),
@@ -87,6 +102,8 @@ _SIGS = {
     "ctr_ramp": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, _P]),
     "ctr_fan_weighting": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P]),
     "ctr_ramp_toeplitz": (C.c_int, [_G, C.c_int, _P, _P, C.c_uint, _P, C.c_int, C.c_int, _P]),
+    "ctr_helical_tables": (C.c_int, [_H, C.c_int, _P]),
+    "ctr_helical": (C.c_int, [_H, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
     "ctr_ragged_fan_bp_plan": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
     "ctr_ragged_fan_bp_run": (C.c_int, [_P, C.c_int, _P]),
     "ctr_ragged_fan_bp_info": (C.c_int, [_P, _I, _I, _I, _I, _D]),

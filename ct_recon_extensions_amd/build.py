@@ -35,7 +35,7 @@ DEVICE_FLAGS = [
 
 CORE_SOURCES = ["tables.cpp", "capi.cpp", "classes.cpp", "fan_bp.hip", "par_bp.hip",
                 "fp_par.hip", "filter.hip", "ragged.hip", "toeplitz.hip", "par_svbp.hip",
-                "fp_rc.hip", "fp_dd.hip"]
+                "fp_rc.hip", "fp_dd.hip", "helical.hip"]
 NUMPY_SOURCES = ["npext.cpp"]
 
 

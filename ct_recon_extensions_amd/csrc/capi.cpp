@@ -36,6 +36,8 @@ int fp_par_t(const ctr_geom& g, int dtype, const void* proj, void* img, int nbat
              hipStream_t st);
 int fp_dd(const ctr_geom& g, int dtype, const void* img, void* proj, int nbatch,
           const double* sc, const double* bo, const int* usex, hipStream_t st);
+int helical(const ctr_helical_geom& g, int mode, int dtype, const void* in, void* out,
+            int nbatch, const double* tables, unsigned int* status, hipStream_t st);
 int fp_dd_grad(const ctr_geom& g, int dtype, const void* proj, void* img, int nbatch,
                const double* w, const double* pos, const int* usex, unsigned int pos_stride,
                hipStream_t st);
@@ -263,6 +265,31 @@ int ctr_fp_dd_grad(const ctr_geom* g, int dtype, const void* proj, void* img, in
     return guarded("ctr_fp_dd_grad", [&] {
         return ctr::fp_dd_grad(*g, dtype, proj, img, nbatch, weights, pos, usex, pos_stride,
                                static_cast<hipStream_t>(stream));
+    });
+}
+
+int ctr_helical(const ctr_helical_geom* g, int mode, int dtype, const void* proj_in,
+                void* proj_out, int nbatch, const double* tables, unsigned int* status,
+                void* stream)
+{
+    CTR_REQUIRE(g && proj_in && proj_out && tables, CTR_ERR_ARG, "ctr_helical: null argument");
+    CTR_REQUIRE(mode == CTR_HELICAL_360 || mode == CTR_HELICAL_180, CTR_ERR_ARG,
+                "ctr_helical: mode must be 360 or 180");
+    CTR_REQUIRE(dtype == CTR_F32 || dtype == CTR_F64, CTR_ERR_ARG, "ctr_helical: bad dtype");
+    CTR_REQUIRE(nbatch >= 1, CTR_ERR_ARG, "ctr_helical: nbatch must be >= 1");
+    CTR_REQUIRE(g->ns >= 1 && g->nt >= 1 && g->na >= 1 && g->view_per_rot >= 1 && g->nz >= 1,
+                CTR_ERR_ARG, "ctr_helical: ns, nt, na, view_per_rot and nz must be >= 1");
+    CTR_REQUIRE(g->ns <= 0x7fffffffu && g->nt <= 0x7fffffffu && g->na <= 0x7fffffffu &&
+                    g->view_per_rot <= 0x7fffffffu && g->nz <= 0x7fffffffu,
+                CTR_ERR_SHAPE, "ctr_helical: sizes must fit in int");
+    auto usable = [](double v) { return std::isfinite(v) && v != 0.0; };
+    CTR_REQUIRE(usable(g->ds) && usable(g->dso) && usable(g->dsd) &&
+                    (mode == CTR_HELICAL_360 || usable(g->orbit)),
+                CTR_ERR_ARG, "ctr_helical: ds, dso, dsd (and orbit for 180LI) must be finite "
+                             "and nonzero");
+    return guarded("ctr_helical", [&] {
+        return ctr::helical(*g, mode, dtype, proj_in, proj_out, nbatch, tables, status,
+                            static_cast<hipStream_t>(stream));
     });
 }
 

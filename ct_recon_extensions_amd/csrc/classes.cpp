@@ -20,6 +20,10 @@
 #include "include/fan_weighting.h"
 #include "include/filter.h"
 #include "include/fp_par_2d.h"
+#include "include/helical_interpolation.h"
+
+#include <stdexcept>
+#include <string>
 
 namespace ct_recon {
 
@@ -708,5 +712,97 @@ template class FanWeighting<float>;
 template class FanWeighting<double>;
 template class FanWeightingGrad<float>;
 template class FanWeightingGrad<double>;
+
+// ----------------------------------------------------- helical rebinning
+namespace {
+
+ctr_helical_geom geom(const HelicalInterpolationParam& p)
+{
+    ctr_helical_geom g{};
+    g.ns = p.ns; g.nt = p.nt; g.na = p.na; g.ds = p.ds; g.dt = p.dt;
+    g.offset_s = p.offset_s; g.offset_t = p.offset_t; g.orbit_start = p.orbit_start;
+    g.orbit = p.orbit; g.couch_begin = p.couch_begin; g.couch_mov = p.couch_mov;
+    g.view_per_rot = p.view_per_rot; g.nz = p.nz; g.dz = p.dz; g.z0 = p.z0;
+    g.thickness = p.thickness; g.dso = p.dso; g.dsd = p.dsd;
+    return g;
+}
+
+// Tables on the host, one small device block for them and the status word,
+// the kernel, then the reference's exceptions from the status.
+template <typename T>
+bool helical_gpu(const HelicalInterpolationParam& p, int mode, const T* in, T* out,
+                 hipStream_t st)
+{
+    const ctr_helical_geom g = geom(p);
+    std::vector<double> tab(3 * static_cast<size_t>(p.ns) + 4 * static_cast<size_t>(p.nz));
+    const int rc = ctr_helical_tables(&g, mode, tab.data());
+    if (rc == CTR_ERR_SHAPE &&
+        std::string(ctr::get_error()).find("View not enough") != std::string::npos)
+        throw std::runtime_error("View not enough for this slice!");
+    if (rc != CTR_OK) return false;
+    const size_t tbytes = tab.size() * sizeof(double);
+    void* dev = nullptr;
+    hipError_t e = hipMalloc(&dev, tbytes + sizeof(unsigned int));
+    if (e != hipSuccess) {
+        ctr::set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return false;
+    }
+    struct Release {
+        void* p;
+        ~Release() { (void)hipFree(p); }
+    } release{dev};
+    unsigned int* status = reinterpret_cast<unsigned int*>(static_cast<char*>(dev) + tbytes);
+    e = hipMemcpyAsync(dev, tab.data(), tbytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(status, 0, sizeof(unsigned int), st);
+    if (e != hipSuccess) {
+        ctr::set_error(std::string("helical tables upload: ") + hipGetErrorString(e));
+        return false;
+    }
+    if (!ok_sync(ctr_helical(&g, mode, dtype_of<T>(), in, out, 1,
+                             static_cast<const double*>(dev), status, st),
+                 st))
+        return false;
+    unsigned int code = 0;
+    e = hipMemcpy(&code, status, sizeof(code), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        ctr::set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+        return false;
+    }
+    if (code & 1u) throw std::runtime_error("No row data available!");
+    if (code & 2u)
+        throw std::runtime_error("180LI conjugate view outside the scan");
+    return true;
+}
+
+}  // namespace
+
+template <typename T>
+bool HelicalInterpolation360<T>::calculate_on_cpu(const T*, T*) const
+{
+    return no_cpu("HelicalInterpolation360::calculate_on_cpu");
+}
+
+template <typename T>
+bool HelicalInterpolation360<T>::calculate_on_gpu(const T* in, T* out, hipStream_t st) const
+{
+    return helical_gpu<T>(this->param_, CTR_HELICAL_360, in, out, st);
+}
+
+template <typename T>
+bool HelicalInterpolation180<T>::calculate_on_cpu(const T*, T*) const
+{
+    return no_cpu("HelicalInterpolation180::calculate_on_cpu");
+}
+
+template <typename T>
+bool HelicalInterpolation180<T>::calculate_on_gpu(const T* in, T* out, hipStream_t st) const
+{
+    return helical_gpu<T>(this->param_, CTR_HELICAL_180, in, out, st);
+}
+
+template class HelicalInterpolation360<float>;
+template class HelicalInterpolation360<double>;
+template class HelicalInterpolation180<float>;
+template class HelicalInterpolation180<double>;
 
 }  // namespace ct_recon

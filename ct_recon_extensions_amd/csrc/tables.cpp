@@ -322,4 +322,59 @@ int ctr_device_info(int* cu_count, int* clock_khz, int* lds_bytes_per_cu, char* 
     return CTR_OK;
 }
 
+int ctr_helical_tables(const ctr_helical_geom* g, int mode, double* tables)
+{
+    CTR_REQUIRE(g && tables, CTR_ERR_ARG, "ctr_helical_tables: null argument");
+    CTR_REQUIRE(mode == CTR_HELICAL_360 || mode == CTR_HELICAL_180, CTR_ERR_ARG,
+                "ctr_helical_tables: mode must be 360 or 180");
+    CTR_REQUIRE(g->ns >= 1 && g->nt >= 1 && g->na >= 1 && g->view_per_rot >= 1 && g->nz >= 1,
+                CTR_ERR_ARG, "ctr_helical_tables: ns, nt, na, view_per_rot and nz must be >= 1");
+    CTR_REQUIRE(g->na <= 0x7fffffffu && g->view_per_rot <= 0x3fffffffu, CTR_ERR_SHAPE,
+                "ctr_helical_tables: na and view_per_rot must fit in int");
+    // per channel (src/helical_interpolation.cc:99-101, 199-201, 215-216):
+    // cos(gamma), the 180LI conjugate's view offset (pi - 2 gamma) / orbit,
+    // and its mirrored channel cs - s/ds
+    const double center_s = (g->ns - 1) / 2.0 + g->offset_s;
+    for (unsigned int is = 0; is < g->ns; ++is) {
+        const double s = (static_cast<int>(is) - center_s) * g->ds;
+        const double gamma = s / g->dsd;
+        tables[is] = cos(gamma);
+        tables[g->ns + is] = (M_PI - 2 * gamma) / g->orbit;
+        tables[2 * g->ns + is] = center_s - s / g->ds;
+    }
+    // slices: z accumulated from z0 (:338-346), the centre view and the
+    // rotation(s) around it (:21-27, :163-168)
+    const int vpr = static_cast<int>(g->view_per_rot);
+    double* sl = tables + 3 * static_cast<size_t>(g->ns);
+    double z = g->z0;
+    for (unsigned int iz = 0; iz < g->nz; ++iz, z += g->dz) {
+        const double c = std::round((z - g->couch_begin) / g->couch_mov);
+        CTR_REQUIRE(std::isfinite(c) && std::fabs(c) < 1e9, CTR_ERR_SHAPE,
+                    "ctr_helical_tables: slice " + std::to_string(iz) +
+                        " has no centre view (couch_mov 0 or z not finite)");
+        const int center_view = static_cast<int>(c);
+        int begin, end, first_view;
+        if (mode == CTR_HELICAL_360) {
+            begin = center_view - vpr;
+            end = center_view + vpr - 1;
+        } else {
+            begin = center_view - vpr / 2;
+            end = begin + vpr - 1;
+        }
+        CTR_REQUIRE(begin >= 0 && end < static_cast<int>(g->na), CTR_ERR_SHAPE,
+                    "ctr_helical_tables: View not enough for this slice! (slice " +
+                        std::to_string(iz) + ", views " + std::to_string(begin) + ".." +
+                        std::to_string(end) + " of " + std::to_string(g->na) + ")");
+        if (mode == CTR_HELICAL_360)
+            first_view = (begin / vpr + 1) * vpr;
+        else
+            first_view = static_cast<int>(std::ceil(begin / static_cast<double>(vpr)) * vpr);
+        sl[4 * iz] = z;
+        sl[4 * iz + 1] = center_view;
+        sl[4 * iz + 2] = begin;
+        sl[4 * iz + 3] = first_view;
+    }
+    return CTR_OK;
+}
+
 }  // extern "C"

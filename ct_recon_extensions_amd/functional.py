@@ -24,14 +24,15 @@ import torch
 
 from . import _lib
 from .geometry import (RC_STEP, FanBackprojection2DAngleParam, FanBackprojection2DParam,
-                       FanWeightingParam, FilterParam, ParallelBackprojection2DParam,
-                       ParallelProjection2DParam, device_tables)
+                       FanWeightingParam, FilterParam, HelicalInterpolationParam,
+                       ParallelBackprojection2DParam, ParallelProjection2DParam, device_tables)
 
 __all__ = [
     "ramp_filter", "ramp_filter_grad", "parallel_backprojection_2d",
     "parallel_backprojection_2d_grad", "parallel_projection_2d", "parallel_projection_2d_grad",
     "fan_backprojection_2d", "fan_backprojection_2d_grad", "fan_backprojection_2d_angle",
     "fan_backprojection_2d_angle_grad", "fan_weighting", "fan_weighting_grad",
+    "helical_interpolation",
 ]
 
 _FILTER_TYPES = {"par": 0, "fan": 1, "flat": 2}
@@ -556,3 +557,51 @@ def fan_filter(proj, proj_shape, channel_space, channel_offset=0, dso=1000, dsd=
     x2 = _rows(proj, p.ns, "proj")
     taps = device_tables("ramp", p, x2.device)
     return _FanFilter.apply(x2, p, taps, wp, impl).reshape(proj.shape)
+
+
+# ------------------------------------------------------- helical rebinning
+def helical_interpolation(proj, param: HelicalInterpolationParam, mode: int = 360,
+                          check: bool = True):
+    """HelicalInterpolation360 (mode 360) or HelicalInterpolation180 (mode
+    180) of include/helical_interpolation.h:86-118 (src/helical_interpolation.cc:
+    81-160, 162-269): a multi-row helical scan ``proj`` of shape
+    [na, nt, ns] or [B, na, nt, ns] (views, detector rows, channels) to nz
+    fan-beam sinograms of one rotation each, [nz, view_per_rot, ns] or
+    [B, nz, view_per_rot, ns], bit-identical to the reference per element.
+
+    A slice without enough views raises ValueError before any work, as the
+    reference throws "View not enough for this slice!".  Where a sample sees
+    no detector row in either window the reference throws "No row data
+    available!": with ``check`` the call synchronises and raises
+    RuntimeError; without it the sample is NaN and nothing synchronises.
+    Not differentiable: the reference defines no gradient for it."""
+    if mode not in (360, 180):
+        raise ValueError("mode must be 360 or 180")
+    _require_gpu(proj, "proj")
+    p = param
+    shape = (p.na, p.nt, p.ns)
+    if tuple(proj.shape[-3:]) != shape or proj.dim() not in (3, 4):
+        raise ValueError(f"proj must be [na, nt, ns] = {list(shape)} or [B, na, nt, ns], "
+                         f"got {list(proj.shape)}")
+    x = proj.contiguous()
+    nb = 1 if x.dim() == 3 else x.shape[0]
+    try:
+        (tables,) = device_tables(f"helical_{mode}", p, x.device)
+    except _lib.CtrError as e:
+        raise ValueError(str(e)) from None
+    out = torch.empty(x.shape[:-3] + (p.nz, p.view_per_rot, p.ns), dtype=x.dtype,
+                      device=x.device)
+    if nb == 0:
+        return out
+    status = torch.zeros(1, dtype=torch.int32, device=x.device) if check else None
+    _lib.call("ctr_helical", C.byref(p.ctr()), int(mode), _dtype_code(x), _ptr(x), _ptr(out),
+              int(nb), _ptr(tables), None if status is None else _ptr(status), _stream(x))
+    if check:
+        code = int(status.item())
+        if code & 1:
+            raise RuntimeError("helical_interpolation: No row data available! (a sample sees "
+                               "no detector row in either window)")
+        if code & 2:
+            raise RuntimeError("helical_interpolation: a 180LI conjugate view lies outside "
+                               "the scan")
+    return out

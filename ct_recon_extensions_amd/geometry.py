@@ -3,9 +3,9 @@
 The dataclasses mirror the reference's Param structs field for field and in
 constructor order (include/bp_fan_2d.h:20-59, include/bp_fan_2d_angle.h:20-55,
 include/bp_par_2d.h:20-55, include/fp_par_2d.h:20-55, include/filter.h:21-36,
-include/fan_weighting.h:21-39).  Tables are built on the host by the
-library's C ABI (the reference's accumulation order, FP64) and cached on the
-device per geometry.
+include/fan_weighting.h:21-39, include/helical_interpolation.h:17-63).
+Tables are built on the host by the library's C ABI (the reference's
+accumulation order, FP64) and cached on the device per geometry.
 """
 from __future__ import annotations
 
@@ -121,6 +121,35 @@ class FanWeightingParam:
         return g
 
 
+@dataclass(frozen=True)
+class HelicalInterpolationParam:
+    """include/helical_interpolation.h:17-63, in its constructor order."""
+    ns: int
+    nt: int
+    na: int
+    ds: float
+    dt: float
+    offset_s: float
+    offset_t: float
+    orbit_start: float
+    orbit: float
+    couch_begin: float
+    couch_mov: float
+    view_per_rot: int
+    nz: int
+    dz: float
+    z0: float
+    thickness: float
+    dso: float
+    dsd: float
+
+    def ctr(self) -> _lib.CtrHelicalGeom:
+        g = _lib.CtrHelicalGeom()
+        for name, _ in _lib.CtrHelicalGeom._fields_:
+            setattr(g, name, getattr(self, name))
+        return g
+
+
 def _geom(p) -> _lib.CtrGeom:
     g = _lib.CtrGeom()
     for name, _ in _lib.CtrGeom._fields_:
@@ -191,6 +220,15 @@ def fp_grad_tables(p: ParallelProjection2DParam, stride: int | None = None):
     return w, pos, usex
 
 
+def helical_tables(p: HelicalInterpolationParam, mode: int):
+    """Per channel cos(gamma), (pi - 2 gamma)/orbit and cs - s/ds, then per
+    slice (z, centre view, window begin, first output view); raises where the
+    reference reports "View not enough for this slice!"."""
+    t = np.zeros(3 * p.ns + 4 * p.nz)
+    _lib.call("ctr_helical_tables", C.byref(p.ctr()), int(mode), _ptr(t))
+    return t
+
+
 def ramp_taps(p: FilterParam):
     h = np.zeros(2 * p.ns + 1)
     _lib.call("ctr_ramp_taps", C.byref(p.ctr()), _ptr(h))
@@ -237,6 +275,8 @@ def device_tables(kind: str, p, device, angles=None):
         host = fp_grad_tables(p)
     elif kind == "ramp":
         host = (ramp_taps(p),)
+    elif kind in ("helical_360", "helical_180"):
+        host = (helical_tables(p, int(kind[-3:])),)
     else:
         raise ValueError(kind)
     dev = tuple(torch.from_numpy(a).to(device) for a in host)

@@ -298,6 +298,45 @@ int ctr_ragged_fan_bp_destroy(ctr_ragged_plan* plan);
 int ctr_ragged_fan_bp_autotune(const ctr_ragged_problem* probs, int nprob, int reps,
                                void* stream, int* best_tile, int* best_vchunk, float* ms_table);
 
+/* ---------------------------------------------------------------------
+ * Helical rebinning: a multi-row helical scan proj_in[b][na][nt][ns]
+ * (view, detector row, channel) to nz fan-beam sinograms
+ * proj_out[b][nz][view_per_rot][ns], one rotation per slice.
+ * ------------------------------------------------------------------- */
+/* HelicalInterpolationParam (include/helical_interpolation.h:17-63) field
+ * for field.  dt and orbit_start are carried but unused, as in the
+ * reference. */
+typedef struct ctr_helical_geom {
+    unsigned int ns, nt, na;
+    double ds, dt, offset_s, offset_t, orbit_start, orbit, couch_begin, couch_mov;
+    unsigned int view_per_rot, nz;
+    double dz, z0, thickness, dso, dsd;
+} ctr_helical_geom;
+
+enum ctr_helical_mode { CTR_HELICAL_360 = 360, CTR_HELICAL_180 = 180 };
+
+/* Host tables of ctr_helical, 3 ns + 4 nz doubles: per channel cos(gamma),
+ * then (pi - 2 gamma) / orbit, then cs - s/ds (the 180LI conjugate's view
+ * offset and mirrored channel), then per slice (z, centre view, first view
+ * of its window, first output view), with z accumulated from z0 in steps of
+ * dz as the reference does (src/helical_interpolation.cc:336-365).  A slice whose window of
+ * +-1 rotation (360) or of one rotation about its centre view (180) leaves
+ * the scan returns CTR_ERR_SHAPE: the reference's "View not enough for this
+ * slice!" (:23-24, :165-166). */
+int ctr_helical_tables(const ctr_helical_geom* g, int mode, double* tables);
+/* HelicalInterpolation360<T> / HelicalInterpolation180<T>::calculate_on_cpu
+ * (include/helical_interpolation.h:86-118; src/helical_interpolation.cc:
+ * 81-160 and 162-269): bit-identical per element in float32 and float64.
+ * tables: device copy of ctr_helical_tables' output.  status (device, may
+ * be NULL) is OR-ed with 1 where a sample has no detector row in either
+ * window (the reference's "No row data available!"; NaN written there) and
+ * with 2 where a 180LI conjugate view would interpolate from a view outside
+ * the scan (the reference reads outside its input; NaN written); clear it
+ * before the call. */
+int ctr_helical(const ctr_helical_geom* g, int mode, int dtype, const void* proj_in,
+                void* proj_out, int nbatch, const double* tables, unsigned int* status,
+                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -731,6 +731,198 @@ DEF_FP_DD_GRAD(float, f)
 DEF_FP_DD_GRAD(double, d)
 
 /* ------------------------------------------------------------------ */
+/* Helical rebinning of a 3-D multi-row sinogram proj_in[na][nt][ns] into
+ * nz 2-D fan sinograms proj_out[nz][view_per_rot][ns]: 360LI
+ * (src/helical_interpolation.cc:81-160, the helper HelicalInterpolation360
+ * uses) and 180LI (:162-269, HelicalInterpolation180).  The geometry record
+ * for these two: ns, nt, na, ds, dt (unused by the reference's arithmetic),
+ * offset_s, offset_t, orbit_start (unused), orbit, couch_begin, couch_mov,
+ * view_per_rot, nz, dz, z0, thickness, dso, dsd.  Slice z advances by
+ * accumulation (z += dz, :304-306).  Per output (slice, view, channel) the
+ * detector rows a slice of that thickness reaches are averaged in the view
+ * (cur) and in the conjugate view (cor: one rotation away for 360LI; the
+ * opposite ray, view cur - (pi - 2 gamma)/orbit with linear interpolation
+ * in view and channel, for 180LI), and the two are interpolated linearly in
+ * z.  A window entirely off the detector is not used.  Returns 1, or where
+ * the reference throws: 0 for a slice whose +-1 rotation (360LI) or half
+ * rotation (180LI) of views is not in the scan (nothing further written),
+ * -1 for a sample with neither window on the detector (NaN there).  -2: a
+ * 180LI conjugate view that interpolates from a view outside the scan (the
+ * reference reads outside its input there; NaN here). */
+typedef struct {
+    unsigned int ns, nt, na;
+    double ds, dt, offset_s, offset_t, orbit_start, orbit, couch_begin, couch_mov;
+    unsigned int view_per_rot, nz;
+    double dz, z0, thickness, dso, dsd;
+} orc_heli;
+
+/* the rows [floor(upper), ceil(bottom)) of the window z_dif -+ thickness/2 */
+static int heli_window(const orc_heli* h, double center_t, double z_dif, double cg,
+                       int* it0, int* it1)
+{
+    double upper_t = center_t - (z_dif + h->thickness / 2.0) / h->dso * cg * h->dsd;
+    double bottom_t = center_t - (z_dif - h->thickness / 2.0) / h->dso * cg * h->dsd;
+    if (bottom_t < 0 || upper_t >= h->nt) return 0;
+    upper_t = upper_t < 0 ? 0 : upper_t;
+    bottom_t = bottom_t >= h->nt ? h->nt : bottom_t;
+    *it0 = (int)floor(upper_t);
+    *it1 = (int)ceil(bottom_t);
+    return 1;
+}
+
+/* 1 when every slice's view window lies in the scan: [cv - vpr, cv + vpr - 1]
+ * for 360LI, [cv - vpr/2, cv - vpr/2 + vpr - 1] for 180LI.  The reference
+ * checks slice by slice and throws at the first short one, after writing
+ * the slices before it; the restatement writes nothing then. */
+static int heli_views_cover(const orc_heli* h, int mode)
+{
+    const int vpr = (int)h->view_per_rot;
+    double z = h->z0;
+    for (unsigned int iz = 0; iz < h->nz; ++iz, z += h->dz) {
+        const int cv = (int)round((z - h->couch_begin) / h->couch_mov);
+        const int begin = mode == 360 ? cv - vpr : cv - vpr / 2;
+        const int end = mode == 360 ? cv + vpr - 1 : begin + vpr - 1;
+        if (begin < 0 || end >= (int)h->na) return 0;
+    }
+    return 1;
+}
+
+#define DEF_HELI360(T, SUF)                                                       \
+int orc_heli360_##SUF(const orc_heli* h, const T* in, T* out)                     \
+{                                                                                 \
+    const int vpr = (int)h->view_per_rot;                                         \
+    const double center_t = (h->nt - 1) / 2.0 + h->offset_t;                     \
+    const double center_s = (h->ns - 1) / 2.0 + h->offset_s;                     \
+    double z = h->z0;                                                             \
+    int ok = 1;                                                                   \
+    if (!heli_views_cover(h, 360)) return 0;                                      \
+    for (unsigned int iz = 0; iz < h->nz; ++iz, z += h->dz) {                     \
+        T* po = out + (size_t)iz * vpr * h->ns;                                   \
+        const int center_view = (int)round((z - h->couch_begin) / h->couch_mov);  \
+        const int begin = center_view - vpr;                                      \
+        const int first_view = (begin / vpr + 1) * vpr;                          \
+        for (int iv = 0; iv < vpr; ++iv) {                                        \
+            const int cur_view = first_view + iv;                                 \
+            const double z_dif = z - h->couch_begin - cur_view * h->couch_mov;    \
+            const int cor_view = cur_view >= center_view ? cur_view - vpr : cur_view + vpr; \
+            const double z_dif2 = z - h->couch_begin - cor_view * h->couch_mov;   \
+            for (int is = 0; is < (int)h->ns; ++is) {                             \
+                const double s = (is - center_s) * h->ds;                         \
+                const double cg = cos(s / h->dsd);                                \
+                double cur = 0, cor = 0;                                          \
+                int a0, a1, use_cur, use_cor, it, cnt;                            \
+                use_cur = heli_window(h, center_t, z_dif, cg, &a0, &a1);          \
+                if (use_cur) {                                                    \
+                    for (cnt = 0, it = a0; it < a1; ++it, ++cnt)                  \
+                        cur += in[is + ((size_t)it + (size_t)cur_view * h->nt) * h->ns]; \
+                    cur /= cnt;                                                   \
+                }                                                                 \
+                use_cor = heli_window(h, center_t, z_dif2, cg, &a0, &a1);         \
+                if (use_cor) {                                                    \
+                    for (cnt = 0, it = a0; it < a1; ++it, ++cnt)                  \
+                        cor += in[is + ((size_t)it + (size_t)cor_view * h->nt) * h->ns]; \
+                    cor /= cnt;                                                   \
+                }                                                                 \
+                T* o = po + is + (size_t)iv * h->ns;                              \
+                if (use_cur && use_cor)                                           \
+                    *o = (T)((z_dif * cor - z_dif2 * cur) / (z_dif - z_dif2));    \
+                else if (use_cur)                                                 \
+                    *o = (T)cur;                                                  \
+                else if (use_cor)                                                 \
+                    *o = (T)cor;                                                  \
+                else {                                                            \
+                    *o = (T)NAN;                                                  \
+                    if (ok > 0) ok = -1;                                          \
+                }                                                                 \
+            }                                                                     \
+        }                                                                         \
+    }                                                                             \
+    return ok;                                                                    \
+}
+DEF_HELI360(float, f)
+DEF_HELI360(double, d)
+
+#define DEF_HELI180(T, SUF)                                                       \
+int orc_heli180_##SUF(const orc_heli* h, const T* in, T* out)                     \
+{                                                                                 \
+    const int vpr = (int)h->view_per_rot;                                         \
+    const double center_t = (h->nt - 1) / 2.0 + h->offset_t;                     \
+    const double center_s = (h->ns - 1) / 2.0 + h->offset_s;                     \
+    double z = h->z0;                                                             \
+    int ok = 1;                                                                   \
+    if (!heli_views_cover(h, 180)) return 0;                                      \
+    for (unsigned int iz = 0; iz < h->nz; ++iz, z += h->dz) {                     \
+        T* po = out + (size_t)iz * vpr * h->ns;                                   \
+        const int center_view = (int)round((z - h->couch_begin) / h->couch_mov);  \
+        const int begin = center_view - vpr / 2, end = begin + vpr - 1;           \
+        const int first_view = (int)(ceil(begin / (double)vpr) * h->view_per_rot); \
+        for (int iv = 0; iv < vpr; ++iv) {                                        \
+            int cur_view = first_view + iv;                                       \
+            if (cur_view > end) cur_view -= vpr;                                  \
+            const double z_dif = z - h->couch_begin - cur_view * h->couch_mov;    \
+            for (int is = 0; is < (int)h->ns; ++is) {                             \
+                const double s = (is - center_s) * h->ds;                         \
+                const double gamma = s / h->dsd;                                  \
+                const double cg = cos(gamma);                                     \
+                double cur = 0, cor = 0, z_dif2, cv;                              \
+                int a0, a1, use_cur, use_cor, it, cnt;                            \
+                T* o = po + is + (size_t)iv * h->ns;                              \
+                use_cur = heli_window(h, center_t, z_dif, cg, &a0, &a1);          \
+                if (use_cur) {                                                    \
+                    for (cnt = 0, it = a0; it < a1; ++it, ++cnt)                  \
+                        cur += in[is + ((size_t)it + (size_t)cur_view * h->nt) * h->ns]; \
+                    cur /= cnt;                                                   \
+                }                                                                 \
+                cv = cur_view - (M_PI - 2 * gamma) / h->orbit;                    \
+                cv = cv < begin ? cv + h->view_per_rot : cv;                      \
+                cv = cv > end ? cv - h->view_per_rot : cv;                        \
+                z_dif2 = z - h->couch_begin - cv * h->couch_mov;                  \
+                use_cor = heli_window(h, center_t, z_dif2, cg, &a0, &a1);         \
+                if (use_cor) {                                                    \
+                    const int vl = (int)floor(cv), vr = (int)ceil(cv);            \
+                    const double u = cv - vl;                                     \
+                    const double ss = center_s - s / h->ds;                       \
+                    int sl, sr;                                                   \
+                    double v, lv = 0, rv = 0;                                     \
+                    if (ss < 0 || ss > h->ns - 1) {                               \
+                        *o = (T)cur;  /* the conjugate channel is off the detector */ \
+                        continue;                                                 \
+                    }                                                             \
+                    if (vl < 0 || vr >= (int)h->na) {                             \
+                        *o = (T)NAN;                                              \
+                        ok = -2;                                                  \
+                        continue;                                                 \
+                    }                                                             \
+                    sl = (int)floor(ss);                                          \
+                    sr = (int)ceil(ss);                                           \
+                    v = ss - sl;                                                  \
+                    for (cnt = 0, it = a0; it < a1; ++it, ++cnt) {                \
+                        const T* rl = in + ((size_t)it + (size_t)vl * h->nt) * h->ns; \
+                        const T* rr = in + ((size_t)it + (size_t)vr * h->nt) * h->ns; \
+                        lv += (1 - v) * rl[sl] + v * rl[sr];                      \
+                        rv += (1 - v) * rr[sl] + v * rr[sr];                      \
+                    }                                                             \
+                    cor = ((1 - u) * lv + u * rv) / cnt;                          \
+                }                                                                 \
+                if (use_cur && use_cor)                                           \
+                    *o = (T)((z_dif * cor - z_dif2 * cur) / (z_dif - z_dif2));    \
+                else if (use_cur)                                                 \
+                    *o = (T)cur;                                                  \
+                else if (use_cor)                                                 \
+                    *o = (T)cor;                                                  \
+                else {                                                            \
+                    *o = (T)NAN;                                                  \
+                    if (ok > 0) ok = -1;                                          \
+                }                                                                 \
+            }                                                                     \
+        }                                                                         \
+    }                                                                             \
+    return ok;                                                                    \
+}
+DEF_HELI180(float, f)
+DEF_HELI180(double, d)
+
+/* ------------------------------------------------------------------ */
 /* Ramp filter taps filter[ns + n], n in [-ns, ns], src/filter.cc:18-65.
  * Returns 0 for an unknown type (:62). */
 int orc_ramp_prep(const orc_geom* g, double* filter)

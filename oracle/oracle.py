@@ -289,3 +289,42 @@ def fanw(g: Geom, x: np.ndarray, grad=False, kind="oracle"):
     out = np.zeros_like(x)
     lib(kind).call("fanw_" + _suffix(x.dtype), g, x, out, int(grad))
     return out
+
+
+# ------------------------------------------------------- helical rebinning
+class Heli(C.Structure):
+    """Mirror of ``orc_heli``: the reference's HelicalInterpolationParam
+    (include/helical_interpolation.h:17-63) in its field order."""
+
+    _fields_ = [
+        ("ns", C.c_uint), ("nt", C.c_uint), ("na", C.c_uint),
+        ("ds", C.c_double), ("dt", C.c_double), ("offset_s", C.c_double),
+        ("offset_t", C.c_double), ("orbit_start", C.c_double), ("orbit", C.c_double),
+        ("couch_begin", C.c_double), ("couch_mov", C.c_double),
+        ("view_per_rot", C.c_uint), ("nz", C.c_uint),
+        ("dz", C.c_double), ("z0", C.c_double), ("thickness", C.c_double),
+        ("dso", C.c_double), ("dsd", C.c_double),
+    ]
+
+
+def heli(**kw) -> Heli:
+    h = Heli()
+    for k, _ in Heli._fields_:
+        setattr(h, k, kw[k])
+    return h
+
+
+def helical(h: Heli, proj: np.ndarray, mode: int, kind="oracle"):
+    """360LI (mode 360, src/helical_interpolation.cc:81-160 via :336-348) or
+    180LI (mode 180, :162-269 via :353-365) of proj[na][nt][ns] into
+    out[nz][view_per_rot][ns].  Returns (out, code): 1, or 0 for a slice
+    short of views and -1 for a sample with no detector row in either window
+    (the reference throws for both), -2 (oracle only) for a 180LI conjugate
+    view interpolated from outside the scan (the reference reads outside its
+    input).  The compiled reference returns 0 for either throw."""
+    proj = np.ascontiguousarray(proj)
+    assert proj.shape == (h.na, h.nt, h.ns)
+    out = np.zeros((h.nz, h.view_per_rot, h.ns), proj.dtype)
+    f = lib(kind).lib[f"{lib(kind).prefix}heli{mode}_{_suffix(proj.dtype)}"]
+    f.restype = C.c_int
+    return out, int(f(C.byref(h), _ptr(proj), _ptr(out)))

@@ -2,7 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
 // the unchanged reference sources src/{bp_fan_2d,bp_fan_2d_angle,bp_par_2d,
-// bp_par_2d_sv,fp_par_2d,filter,fan_weighting}.cc, read where they lie under the reference
+// bp_par_2d_sv,fp_par_2d,filter,fan_weighting,
+// helical_interpolation}.cc, read where they lie under the reference
 // checkout, into oracle/_ref/libctref.so.  Nothing of the reference is copied
 // into this repository; the built library is git-ignored.  It is used to
 // write golden vectors (tests/golden/make_golden.py), to cross-check the C
@@ -18,6 +19,9 @@
 #include "include/fp_par_2d.h"
 #include "include/filter.h"
 #include "include/fan_weighting.h"
+#include "include/helical_interpolation.h"
+
+#include <stdexcept>
 
 using namespace ct_recon;
 
@@ -303,6 +307,58 @@ int ref_fanw_d(const orc_geom* g, const double* in, double* out, int grad)
 {
     return grad ? FanWeightingGrad<double>(fanw_param(g)).calculate_on_cpu(in, out)
                 : FanWeighting<double>(fanw_param(g)).calculate_on_cpu(in, out);
+}
+
+// Helical rebinning: the record is orc_heli of oracle/ct_oracle.c.  The
+// reference throws std::runtime_error for a slice without enough views or a
+// sample with no detector row in either window; both return 0 here.
+struct orc_heli {
+    unsigned int ns, nt, na;
+    double ds, dt, offset_s, offset_t, orbit_start, orbit, couch_begin, couch_mov;
+    unsigned int view_per_rot, nz;
+    double dz, z0, thickness, dso, dsd;
+};
+
+static HelicalInterpolationParam heli_param(const orc_heli* h)
+{
+    return HelicalInterpolationParam(h->ns, h->nt, h->na, h->ds, h->dt, h->offset_s, h->offset_t,
+                                     h->orbit_start, h->orbit, h->couch_begin, h->couch_mov,
+                                     h->view_per_rot, h->nz, h->dz, h->z0, h->thickness, h->dso,
+                                     h->dsd);
+}
+
+}  // extern "C"
+
+template <class Op, typename T>
+static int heli_run(const orc_heli* h, const T* in, T* out)
+{
+    try {
+        return Op(heli_param(h)).calculate_on_cpu(in, out) ? 1 : 0;
+    } catch (const std::runtime_error&) {
+        return 0;
+    }
+}
+
+extern "C" {
+
+int ref_heli360_f(const orc_heli* h, const float* in, float* out)
+{
+    return heli_run<HelicalInterpolation360<float>>(h, in, out);
+}
+
+int ref_heli360_d(const orc_heli* h, const double* in, double* out)
+{
+    return heli_run<HelicalInterpolation360<double>>(h, in, out);
+}
+
+int ref_heli180_f(const orc_heli* h, const float* in, float* out)
+{
+    return heli_run<HelicalInterpolation180<float>>(h, in, out);
+}
+
+int ref_heli180_d(const orc_heli* h, const double* in, double* out)
+{
+    return heli_run<HelicalInterpolation180<double>>(h, in, out);
 }
 
 }  // extern "C"

@@ -3,6 +3,7 @@
 // tests/test_class_api.py.  Usage:
 //   class_api_main cpu <dir>   tables from every Prep::calculate_on_cpu
 //   class_api_main gpu <dir>   every operator on the GPU, float and double
+//                              (helical rebinning included)
 // Inputs are read from <dir>/*.in.f64 and outputs written to <dir>/*.f64 as
 // raw little-endian doubles; geometries are fixed and mirrored in the test.
 #include <hip/hip_runtime.h>
@@ -20,6 +21,9 @@
 #include "include/fan_weighting.h"
 #include "include/filter.h"
 #include "include/fp_par_2d.h"
+#include "include/helical_interpolation.h"
+
+#include <stdexcept>
 
 using namespace ct_recon;
 
@@ -221,11 +225,47 @@ static int gpu_typed(const char* suf, hipStream_t st)
     return 0;
 }
 
+// Helical rebinning; the geometry comes from <dir>/heli_geom.in.f64 (the
+// 18 fields in constructor order), the scan from heli_proj.in.f64.
+static HelicalInterpolationParam heli_geom(const std::vector<double>& v)
+{
+    return HelicalInterpolationParam(
+        static_cast<unsigned>(v[0]), static_cast<unsigned>(v[1]), static_cast<unsigned>(v[2]),
+        v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], static_cast<unsigned>(v[11]),
+        static_cast<unsigned>(v[12]), v[13], v[14], v[15], v[16], v[17]);
+}
+
+template <typename T>
+static int heli_typed(const char* suf, hipStream_t st)
+{
+    const std::string s = suf;
+    const HelicalInterpolationParam p = heli_geom(load("heli_geom", 18));
+    Dev<T> in(load("heli_proj", static_cast<size_t>(p.na) * p.nt * p.ns));
+    Dev<T> out(static_cast<size_t>(p.nz) * p.view_per_rot * p.ns);
+    CHECK(HelicalInterpolation360<T>(p).calculate_on_gpu(in.p, out.p, st));
+    save("heli360_" + s, out.get());
+    CHECK(HelicalInterpolation180<T>(p).calculate_on_gpu(in.p, out.p, st));
+    save("heli180_" + s, out.get());
+    CHECK(!HelicalInterpolation360<T>(p).calculate_on_cpu(nullptr, nullptr));
+    // a slice one view too close to the scan start: the reference's exception
+    HelicalInterpolationParam q = p;
+    q.z0 = p.couch_begin + (p.view_per_rot - 1) * p.couch_mov;
+    bool threw = false;
+    try {
+        HelicalInterpolation360<T>(q).calculate_on_gpu(in.p, out.p, st);
+    } catch (const std::runtime_error& e) {
+        threw = std::strcmp(e.what(), "View not enough for this slice!") == 0;
+    }
+    CHECK(threw);
+    return 0;
+}
+
 static int gpu()
 {
     hipStream_t st;
     CHECK(hipStreamCreate(&st) == hipSuccess);
     if (gpu_typed<float>("f32", st) || gpu_typed<double>("f64", st)) return 1;
+    if (heli_typed<float>("f32", st) || heli_typed<double>("f64", st)) return 1;
     CHECK(hipStreamDestroy(st) == hipSuccess);
     std::puts("gpu ok");
     return 0;

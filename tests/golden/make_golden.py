@@ -208,6 +208,47 @@ def filter_fixtures():
          **out)
 
 
+# Helical rebinning (src/helical_interpolation.cc): a pitch-1 scan whose
+# windows leave the detector near +-1 rotation (the single-window branch) and
+# whose channel offset puts 180LI conjugate channels off the detector; a
+# pitch-1.7 scan with thick slices (several rows averaged), a row offset and
+# descending slices; a pitch-3 scan with thin slices where some samples see
+# no row in either window (the reference throws); a slice too close to the
+# scan start (the reference throws).  The projections are regenerated from
+# proj_seed (float32 draws, cast for the float64 run).
+HELI_COLL = 6 * 595.0 / 1085.6  # 6 rows of 1 mm at the detector, at iso
+HELI_SMALL = dict(ns=29, nt=6, na=96, ds=12.0, dt=1.0, offset_s=0.3, offset_t=0.15,
+                  orbit_start=0.0, orbit=2 * math.pi / 24, couch_begin=-5.0,
+                  couch_mov=HELI_COLL / 24, view_per_rot=24, nz=4, dz=0.9,
+                  z0=-5.0 + 30 * HELI_COLL / 24, thickness=1.5, dso=595.0, dsd=1085.6)
+HELI_PITCH = dict(ns=23, nt=5, na=80, ds=15.0, dt=1.0, offset_s=-0.7, offset_t=-0.4,
+                  orbit_start=0.0, orbit=2 * math.pi / 20, couch_begin=12.0,
+                  couch_mov=1.7 * 5 * 595.0 / 1085.6 / 20, view_per_rot=20, nz=5, dz=-0.8,
+                  z0=12.0 + 52 * 1.7 * 5 * 595.0 / 1085.6 / 20, thickness=3.0, dso=595.0,
+                  dsd=1085.6)
+HELI_GAP = dict(HELI_SMALL, couch_mov=3 * HELI_COLL / 24, z0=-5.0 + 30 * 3 * HELI_COLL / 24,
+                dz=2.0, thickness=0.2)
+HELI_SHORT = dict(HELI_SMALL, z0=-5.0 + 20 * HELI_COLL / 24)
+
+
+def heli_fixtures():
+    for name, geo, seed in (("heli_small", HELI_SMALL, 101), ("heli_pitch", HELI_PITCH, 103),
+                            ("heli_gap", HELI_GAP, 105), ("heli_short", HELI_SHORT, 107)):
+        h = O.heli(**geo)
+        proj = randn(seed, (h.na, h.nt, h.ns))
+        out = {}
+        for mode in (360, 180):
+            for suf, x in (("f32", proj), ("f64", proj.astype(np.float64))):
+                y, ok = O.helical(h, x, mode, kind="ref")
+                out[f"ok_{mode}_{suf}"] = np.int64(ok)
+                if ok:
+                    out[f"out_{mode}_{suf}"] = y
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, geom=json.dumps(geo), proj_seed=np.int64(seed), **out)
+        print(f"wrote {path} ({os.path.getsize(path)} B)",
+              {k: int(v) for k, v in out.items() if k.startswith("ok")})
+
+
 if __name__ == "__main__":
     if not O.have_ref():
         sys.exit("oracle/_ref/libctref.so missing: run `make -C oracle ref` first")
@@ -218,3 +259,4 @@ if __name__ == "__main__":
     fp_fixtures()
     fpdd_fixtures()
     filter_fixtures()
+    heli_fixtures()

@@ -22,6 +22,10 @@ FP = dict(ns=41, na=40, ds=1.0, orbit=math.pi / 40, offset_s=0.3, orbit_start=0.
           dx=1.0, dy=1.0, offset_x=0.2, offset_y=-0.1)
 FILT = dict(ns=33, nrow=5, ds=0.9, dsd=40.0, type=1)
 FW = dict(ns=37, nrow=4, ds=1.3, offset_s=0.6, dso=595.0, dsd=1085.6, type=1)
+# Helical rebinning: passed to the driver through heli_geom.in.f64.
+HELI = dict(ns=23, nt=5, na=80, ds=15.0, dt=1.0, offset_s=-0.7, offset_t=-0.4, orbit_start=0.0,
+            orbit=2 * math.pi / 20, couch_begin=12.0, couch_mov=0.2, view_per_rot=20, nz=5,
+            dz=-0.8, z0=22.4, thickness=3.0, dso=595.0, dsd=1085.6)
 
 
 def _read(d, name):
@@ -74,6 +78,8 @@ def test_class_api_operators_gpu(driver, tmp_path):
         "fp_proj": randn(7, (gq.na, gq.ns), np.float64),
         "ramp_in": randn(8, (FILT["nrow"], FILT["ns"]), np.float64),
         "fw_in": randn(9, (FW["nrow"], FW["ns"]), np.float64),
+        "heli_geom": np.array([float(HELI[k]) for k, _ in O.Heli._fields_]),
+        "heli_proj": randn(10, (HELI["na"], HELI["nt"], HELI["ns"]), np.float64),
     }
     for k, v in inputs.items():
         _write(tmp_path, k, v)
@@ -115,6 +121,12 @@ def test_class_api_operators_gpu(driver, tmp_path):
             _read(tmp_path, f"fpddg_{suf}"),
             O.fp_dd_grad(gq, inputs["fp_proj"].astype(dt),
                          O.fp_dd_grad_prep(gq)).ravel().astype(np.float64))
+        hh = O.heli(**HELI)
+        for mode in (360, 180):
+            want, code = O.helical(hh, inputs["heli_proj"].astype(dt), mode)
+            assert code == 1
+            np.testing.assert_array_equal(_read(tmp_path, f"heli{mode}_{suf}"),
+                                          want.ravel().astype(np.float64))
         for name, want in ref.items():
             got = _read(tmp_path, f"{name}_{suf}")
             t = 1e-5 if suf == "f32" else tol
