@@ -421,11 +421,13 @@ def test_autograd_fan_fbp_backward_is_transpose(F, det):
 
 
 # ------------------------------------------- ramp filter on the matrix cores
-# 37 rows: a partial last row block.  The whole-row BF16x3 kernel takes rows
-# of up to 1024 channels (three workgroups per CU up to 763, two beyond);
-# 1025 and longer take the tiled one.
+# 37 rows: a partial last row block.  The BF16x3 filter takes whole rows in
+# LDS: the four-wave kernel up to 1024 channels (three workgroups per CU up
+# to 763, two beyond), the persistent eight-wave one up to 1536; longer rows
+# take the tiled kernel.
 @pytest.mark.parametrize("impl", ["mfma", "bf16x3"])
-@pytest.mark.parametrize("ns", [2, 3, 33, 64, 367, 736, 763, 764, 927, 1024, 1025, 1472])
+@pytest.mark.parametrize("ns", [2, 3, 33, 64, 367, 736, 763, 764, 927, 1024, 1025, 1280, 1281,
+                                1472, 1536, 1537])
 @pytest.mark.parametrize("typ,name", [(0, "par"), (1, "fan")])
 def test_ramp_matrix_cores_vs_oracle(F, ns, typ, name, impl):
     nrow = 37
@@ -434,6 +436,22 @@ def test_ramp_matrix_cores_vs_oracle(F, ns, typ, name, impl):
     ref = O.ramp(g, x, O.ramp_prep(g))
     out = host(F.ramp_filter(cuda(x), ns, nrow, 0.65, 1085.6, name, impl=impl))
     assert rel_l2(out, ref) < TOL32
+
+
+@pytest.mark.parametrize("ns", [736, 1000, 1472])
+def test_ramp_bf16x3_many_row_blocks(F, ns):
+    """More 16-row blocks than workgroups, so the persistent kernels walk
+    several blocks each with the next block's rows prefetched, and a partial
+    last block: rows from the first, a middle and the last blocks against
+    the oracle."""
+    nrow = 16 * 1100 + 5
+    g = O.geom(ns=ns, nrow=nrow, ds=0.65, dsd=1085.6, type=1)
+    x = randn(ns + 5, (nrow, ns))
+    out = host(F.ramp_filter(cuda(x), ns, nrow, 0.65, 1085.6, "fan", impl="bf16x3"))
+    h = O.ramp_prep(g)
+    for r0, r1 in ((0, 20), (16 * 537 - 3, 16 * 537 + 19), (nrow - 21, nrow)):
+        gs = O.geom(ns=ns, nrow=r1 - r0, ds=0.65, dsd=1085.6, type=1)
+        assert rel_l2(out[r0:r1], O.ramp(gs, x[r0:r1], h)) < TOL32, (r0, r1)
 
 
 def test_ramp_bf16x3_precision(F):
