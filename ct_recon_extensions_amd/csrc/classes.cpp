@@ -4,8 +4,9 @@
 // the caller's device buffers.  Operator classes run one slice on the given
 // stream and synchronise it before returning, like the reference's
 // calculate_on_gpu (e.g. cuda/bp_fan_2d.cu:229).  The operators'
-// calculate_on_cpu is the reference's CPU code, which this GPU library does
-// not provide: it returns false and says so through ctr_last_error().
+// calculate_on_cpu takes host buffers, as the reference's does
+// (e.g. src/bp_fan_2d.cc:46-91), and computes on the GPU: the buffers are
+// staged through device memory and the same kernels run.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -33,13 +34,6 @@ template <typename T>
 constexpr int dtype_of()
 {
     return std::is_same<T, double>::value ? CTR_F64 : CTR_F32;
-}
-
-bool no_cpu(const char* who)
-{
-    ctr::set_error(std::string(who) +
-                   ": the MI355X library has no CPU implementation; use calculate_on_gpu");
-    return false;
 }
 
 bool ok_sync(int rc, hipStream_t st)
@@ -153,6 +147,83 @@ struct DevTemp {
     }
 };
 
+// Host-memory calls (the operators' calculate_on_cpu): the library computes
+// only on the GPU, so these stage the caller's host buffers through device
+// memory on a stream of their own and run the same kernels as
+// calculate_on_gpu, whose result they return bit for bit.  Every buffer,
+// outputs included, is copied in first, so the operator sees exactly what a
+// device caller would hand it.
+class Staging {
+public:
+    Staging()
+    {
+        const hipError_t e = hipStreamCreate(&st_);
+        if (e != hipSuccess) {
+            ctr::set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+            st_ = nullptr;
+        }
+    }
+    ~Staging()
+    {
+        if (st_) (void)hipStreamSynchronize(st_);
+        for (void* p : bufs_) (void)hipFree(p);
+        if (st_) (void)hipStreamDestroy(st_);
+    }
+    bool ok() const { return st_ != nullptr && ok_; }
+    hipStream_t stream() const { return st_; }
+    // device copy of host[0..n); null in, null out (the operator reports it)
+    template <typename V>
+    V* in(const V* host, size_t n)
+    {
+        if (!host || !ok()) return nullptr;
+        void* d = alloc(n * sizeof(V));
+        if (!d) return nullptr;
+        const hipError_t e = hipMemcpyAsync(d, host, n * sizeof(V), hipMemcpyHostToDevice, st_);
+        if (e != hipSuccess) return fail("hipMemcpyAsync", e);
+        return static_cast<V*>(d);
+    }
+    // copy the device buffer back into host[0..n) (synchronises)
+    template <typename V>
+    bool out(V* host, const V* dev, size_t n)
+    {
+        if (!host || !dev || !ok()) return false;
+        hipError_t e = hipMemcpyAsync(host, dev, n * sizeof(V), hipMemcpyDeviceToHost, st_);
+        if (e == hipSuccess) e = hipStreamSynchronize(st_);
+        if (e != hipSuccess) {
+            fail("hipMemcpyAsync", e);
+            return false;
+        }
+        return true;
+    }
+
+private:
+    void* alloc(size_t bytes)
+    {
+        void* d = nullptr;
+        const hipError_t e = hipMalloc(&d, bytes ? bytes : 1);
+        if (e != hipSuccess) return fail("hipMalloc", e);
+        bufs_.push_back(d);
+        return d;
+    }
+    std::nullptr_t fail(const char* what, hipError_t e)
+    {
+        ctr::set_error(std::string(what) + ": " + hipGetErrorString(e));
+        ok_ = false;
+        return nullptr;
+    }
+    hipStream_t st_ = nullptr;
+    bool ok_ = true;
+    std::vector<void*> bufs_;
+};
+
+// Runs `gpu` (the class's calculate_on_gpu on staged pointers) and copies the
+// output back.
+template <typename V, typename F>
+bool staged(Staging& s, V* host_out, V* dev_out, size_t n_out, F&& gpu)
+{
+    return s.ok() && dev_out && gpu() && s.out(host_out, dev_out, n_out);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- bp_fan_2d
@@ -183,10 +254,19 @@ bool FanBackprojection2DPixDrivenGradPrep::calculate_on_gpu(double* xpos, double
 }
 
 template <typename T>
-bool FanBackprojection2DPixDriven<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                       const double*, const double*) const
+bool FanBackprojection2DPixDriven<T>::calculate_on_cpu(const T* proj, T* img, const double* xpos,
+                                                       const double* ypos,
+                                                       const double* sc) const
 {
-    return no_cpu("FanBackprojection2DPixDriven::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t ni = static_cast<size_t>(p.nx) * p.ny;
+    Staging s;
+    T* d = s.in(img, ni);
+    return staged(s, img, d, ni, [&] {
+        return calculate_on_gpu(s.in(proj, static_cast<size_t>(p.na) * p.ns), d,
+                                s.in(xpos, p.nx), s.in(ypos, p.ny), s.in(sc, 2 * p.na),
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -201,10 +281,19 @@ bool FanBackprojection2DPixDriven<T>::calculate_on_gpu(const T* proj, T* img, co
 }
 
 template <typename T>
-bool FanBackprojection2DPixDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                           const double*, const double*) const
+bool FanBackprojection2DPixDrivenGrad<T>::calculate_on_cpu(const T* img, T* grad,
+                                                           const double* xpos, const double* ypos,
+                                                           const double* sc) const
 {
-    return no_cpu("FanBackprojection2DPixDrivenGrad::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t np = static_cast<size_t>(p.na) * p.ns;
+    Staging s;
+    T* d = s.in(grad, np);
+    return staged(s, grad, d, np, [&] {
+        return calculate_on_gpu(s.in(img, static_cast<size_t>(p.nx) * p.ny), d,
+                                s.in(xpos, p.nx), s.in(ypos, p.ny), s.in(sc, 2 * p.na),
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -255,10 +344,20 @@ bool FanBackprojection2DAnglePixDrivenGradPrep::calculate_on_gpu(const double* a
 }
 
 template <typename T>
-bool FanBackprojection2DAnglePixDriven<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                            const double*, const double*) const
+bool FanBackprojection2DAnglePixDriven<T>::calculate_on_cpu(const T* proj, T* img,
+                                                            const double* xpos,
+                                                            const double* ypos,
+                                                            const double* sc) const
 {
-    return no_cpu("FanBackprojection2DAnglePixDriven::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t ni = static_cast<size_t>(p.nx) * p.ny;
+    Staging s;
+    T* d = s.in(img, ni);
+    return staged(s, img, d, ni, [&] {
+        return calculate_on_gpu(s.in(proj, static_cast<size_t>(p.na) * p.ns), d,
+                                s.in(xpos, p.nx), s.in(ypos, p.ny), s.in(sc, 2 * p.na),
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -273,11 +372,20 @@ bool FanBackprojection2DAnglePixDriven<T>::calculate_on_gpu(const T* proj, T* im
 }
 
 template <typename T>
-bool FanBackprojection2DAnglePixDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                                const double*,
-                                                                const double*) const
+bool FanBackprojection2DAnglePixDrivenGrad<T>::calculate_on_cpu(const T* img, T* grad,
+                                                                const double* xpos,
+                                                                const double* ypos,
+                                                                const double* sc) const
 {
-    return no_cpu("FanBackprojection2DAnglePixDrivenGrad::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t np = static_cast<size_t>(p.na) * p.ns;
+    Staging s;
+    T* d = s.in(grad, np);
+    return staged(s, grad, d, np, [&] {
+        return calculate_on_gpu(s.in(img, static_cast<size_t>(p.nx) * p.ny), d,
+                                s.in(xpos, p.nx), s.in(ypos, p.ny), s.in(sc, 2 * p.na),
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -380,10 +488,21 @@ bool ParallelSingleViewBp2DPixDrivenPrep::calculate_on_gpu(double* xcos, double*
 }
 
 template <typename T>
-bool ParallelSingleViewBp2DPixDriven<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                          const double*, const int*) const
+bool ParallelSingleViewBp2DPixDriven<T>::calculate_on_cpu(const T* proj, T* img,
+                                                          const double* xcos,
+                                                          const double* ysin,
+                                                          const int*) const
 {
-    return no_cpu("ParallelSingleViewBp2DPixDriven::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t ni = static_cast<size_t>(p.nx) * p.ny * p.na;
+    Staging s;
+    T* d = s.in(img, ni);
+    return staged(s, img, d, ni, [&] {
+        return calculate_on_gpu(s.in(proj, static_cast<size_t>(p.na) * p.ns), d,
+                                s.in(xcos, static_cast<size_t>(p.na) * p.nx),
+                                s.in(ysin, static_cast<size_t>(p.na) * p.ny), nullptr,
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -400,10 +519,21 @@ template class ParallelSingleViewBp2DPixDriven<float>;
 template class ParallelSingleViewBp2DPixDriven<double>;
 
 template <typename T>
-bool ParallelBackprojection2DPixDriven<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                            const double*, const int*) const
+bool ParallelBackprojection2DPixDriven<T>::calculate_on_cpu(const T* proj, T* img,
+                                                            const double* xcos,
+                                                            const double* ysin,
+                                                            const int*) const
 {
-    return no_cpu("ParallelBackprojection2DPixDriven::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t ni = static_cast<size_t>(p.nx) * p.ny;
+    Staging s;
+    T* d = s.in(img, ni);
+    return staged(s, img, d, ni, [&] {
+        return calculate_on_gpu(s.in(proj, static_cast<size_t>(p.na) * p.ns), d,
+                                s.in(xcos, static_cast<size_t>(p.na) * p.nx),
+                                s.in(ysin, static_cast<size_t>(p.na) * p.ny), nullptr,
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -416,12 +546,20 @@ bool ParallelBackprojection2DPixDriven<T>::calculate_on_gpu(const T* proj, T* im
     return ok_sync(ctr_par_bp(&g, dtype_of<T>(), proj, img, 1, xcos, ysin, st), st);
 }
 
+// The GPU transpose builds its own tables, so the Grad Prep's are not staged.
 template <typename T>
-bool ParallelBackprojection2DPixDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                                const double*,
+bool ParallelBackprojection2DPixDrivenGrad<T>::calculate_on_cpu(const T* img, T* grad,
+                                                                const double*, const double*,
                                                                 const bool*) const
 {
-    return no_cpu("ParallelBackprojection2DPixDrivenGrad::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t np = static_cast<size_t>(p.na) * p.ns;
+    Staging s;
+    T* d = s.in(grad, np);
+    return staged(s, grad, d, np, [&] {
+        return calculate_on_gpu(s.in(img, static_cast<size_t>(p.nx) * p.ny), d, nullptr,
+                                nullptr, nullptr, s.stream());
+    });
 }
 
 // The exact transpose needs the forward tables, built here per call.
@@ -484,10 +622,19 @@ bool ParallelProjection2DRayDrivenGradPrep::calculate_on_gpu(double* w, double* 
 }
 
 template <typename T>
-bool ParallelProjection2DRayDriven<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                        const double*, const int*) const
+bool ParallelProjection2DRayDriven<T>::calculate_on_cpu(const T* img, T* proj, const double* sc,
+                                                        const double* bo,
+                                                        const int* usex) const
 {
-    return no_cpu("ParallelProjection2DRayDriven::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t np = static_cast<size_t>(p.na) * p.ns;
+    Staging s;
+    T* d = s.in(proj, np);
+    return staged(s, proj, d, np, [&] {
+        return calculate_on_gpu(s.in(img, static_cast<size_t>(p.nx) * p.ny), d,
+                                s.in(sc, 2 * p.na), s.in(bo, 2 * np), s.in(usex, p.na),
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -500,10 +647,19 @@ bool ParallelProjection2DRayDriven<T>::calculate_on_gpu(const T* img, T* proj, c
 }
 
 template <typename T>
-bool ParallelProjection2DRayDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                            const double*, const int*) const
+bool ParallelProjection2DRayDrivenGrad<T>::calculate_on_cpu(const T* proj, T* img,
+                                                            const double* w, const double* pos,
+                                                            const int* usex) const
 {
-    return no_cpu("ParallelProjection2DRayDrivenGrad::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t ni = static_cast<size_t>(p.nx) * p.ny;
+    Staging s;
+    T* d = s.in(img, ni);
+    return staged(s, img, d, ni, [&] {
+        return calculate_on_gpu(s.in(proj, static_cast<size_t>(p.na) * p.ns), d, s.in(w, p.na),
+                                s.in(pos, 2 * static_cast<size_t>(p.na) * p.nx),
+                                s.in(usex, p.na), s.stream());
+    });
 }
 
 template <typename T>
@@ -541,10 +697,19 @@ bool ParallelProjection2DRayCastingPrepare::calculate_on_gpu(double* sc, double*
 }
 
 template <typename T>
-bool ParallelProjection2DRayCasting<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                         const double*, const int*) const
+bool ParallelProjection2DRayCasting<T>::calculate_on_cpu(const T* img, T* proj, const double* sc,
+                                                         const double* begins,
+                                                         const int* nsteps) const
 {
-    return no_cpu("ParallelProjection2DRayCasting::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t np = static_cast<size_t>(p.na) * p.ns;
+    Staging s;
+    T* d = s.in(proj, np);
+    return staged(s, proj, d, np, [&] {
+        return calculate_on_gpu(s.in(img, static_cast<size_t>(p.nx) * p.ny), d,
+                                s.in(sc, 2 * p.na), s.in(begins, 2 * np), s.in(nsteps, np),
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -597,10 +762,19 @@ bool ParallelProjection2DDisDrivenGradPrep::calculate_on_gpu(double* w, double* 
 }
 
 template <typename T>
-bool ParallelProjection2DDisDriven<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                        const double*, const int*) const
+bool ParallelProjection2DDisDriven<T>::calculate_on_cpu(const T* img, T* proj, const double* sc,
+                                                        const double* bo,
+                                                        const int* usex) const
 {
-    return no_cpu("ParallelProjection2DDisDriven::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t np = static_cast<size_t>(p.na) * p.ns;
+    Staging s;
+    T* d = s.in(proj, np);
+    return staged(s, proj, d, np, [&] {
+        return calculate_on_gpu(s.in(img, static_cast<size_t>(p.nx) * p.ny), d,
+                                s.in(sc, 2 * p.na), s.in(bo, 3 * np), s.in(usex, p.na),
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -613,10 +787,19 @@ bool ParallelProjection2DDisDriven<T>::calculate_on_gpu(const T* img, T* proj, c
 }
 
 template <typename T>
-bool ParallelProjection2DDisDrivenGrad<T>::calculate_on_cpu(const T*, T*, const double*,
-                                                            const double*, const int*) const
+bool ParallelProjection2DDisDrivenGrad<T>::calculate_on_cpu(const T* proj, T* img,
+                                                            const double* w, const double* pos,
+                                                            const int* usex) const
 {
-    return no_cpu("ParallelProjection2DDisDrivenGrad::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t ni = static_cast<size_t>(p.nx) * p.ny;
+    Staging s;
+    T* d = s.in(img, ni);
+    return staged(s, img, d, ni, [&] {
+        return calculate_on_gpu(s.in(proj, static_cast<size_t>(p.na) * p.ns), d, s.in(w, p.na),
+                                s.in(pos, 2 * static_cast<size_t>(p.na) * p.nx),
+                                s.in(usex, p.na), s.stream());
+    });
 }
 
 template <typename T>
@@ -649,9 +832,15 @@ bool RampFilterPrep::calculate_on_gpu(double* filter, hipStream_t st) const
 }
 
 template <typename T>
-bool RampFilter<T>::calculate_on_cpu(const T*, const double*, T*) const
+bool RampFilter<T>::calculate_on_cpu(const T* in, const double* filter, T* out) const
 {
-    return no_cpu("RampFilter::calculate_on_cpu");
+    const size_t n = static_cast<size_t>(param_.nrow) * param_.ns;
+    Staging s;
+    T* d = s.in(out, n);
+    return staged(s, out, d, n, [&] {
+        return calculate_on_gpu(s.in(in, n), s.in(filter, 2 * static_cast<size_t>(param_.ns) + 1),
+                                d, s.stream());
+    });
 }
 
 template <typename T>
@@ -663,9 +852,15 @@ bool RampFilter<T>::calculate_on_gpu(const T* in, const double* filter, T* out,
 }
 
 template <typename T>
-bool RampFilterGrad<T>::calculate_on_cpu(const T*, const double*, T*) const
+bool RampFilterGrad<T>::calculate_on_cpu(const T* in, const double* filter, T* out) const
 {
-    return no_cpu("RampFilterGrad::calculate_on_cpu");
+    const size_t n = static_cast<size_t>(param_.nrow) * param_.ns;
+    Staging s;
+    T* d = s.in(out, n);
+    return staged(s, out, d, n, [&] {
+        return calculate_on_gpu(s.in(in, n), s.in(filter, 2 * static_cast<size_t>(param_.ns) + 1),
+                                d, s.stream());
+    });
 }
 
 template <typename T>
@@ -683,9 +878,12 @@ template class RampFilterGrad<double>;
 
 // --------------------------------------------------------- fan_weighting
 template <typename T>
-bool FanWeighting<T>::calculate_on_cpu(const T*, T*) const
+bool FanWeighting<T>::calculate_on_cpu(const T* in, T* out) const
 {
-    return no_cpu("FanWeighting::calculate_on_cpu");
+    const size_t n = static_cast<size_t>(param_.nrow) * param_.ns;
+    Staging s;
+    T* d = s.in(out, n);
+    return staged(s, out, d, n, [&] { return calculate_on_gpu(s.in(in, n), d, s.stream()); });
 }
 
 template <typename T>
@@ -696,9 +894,12 @@ bool FanWeighting<T>::calculate_on_gpu(const T* in, T* out, hipStream_t st) cons
 }
 
 template <typename T>
-bool FanWeightingGrad<T>::calculate_on_cpu(const T*, T*) const
+bool FanWeightingGrad<T>::calculate_on_cpu(const T* in, T* out) const
 {
-    return no_cpu("FanWeightingGrad::calculate_on_cpu");
+    const size_t n = static_cast<size_t>(param_.nrow) * param_.ns;
+    Staging s;
+    T* d = s.in(out, n);
+    return staged(s, out, d, n, [&] { return calculate_on_gpu(s.in(in, n), d, s.stream()); });
 }
 
 template <typename T>
@@ -777,9 +978,16 @@ bool helical_gpu(const HelicalInterpolationParam& p, int mode, const T* in, T* o
 }  // namespace
 
 template <typename T>
-bool HelicalInterpolation360<T>::calculate_on_cpu(const T*, T*) const
+bool HelicalInterpolation360<T>::calculate_on_cpu(const T* in, T* out) const
 {
-    return no_cpu("HelicalInterpolation360::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t no = static_cast<size_t>(p.nz) * p.view_per_rot * p.ns;
+    Staging s;
+    T* d = s.in(out, no);
+    return staged(s, out, d, no, [&] {
+        return calculate_on_gpu(s.in(in, static_cast<size_t>(p.na) * p.nt * p.ns), d,
+                                s.stream());
+    });
 }
 
 template <typename T>
@@ -789,9 +997,16 @@ bool HelicalInterpolation360<T>::calculate_on_gpu(const T* in, T* out, hipStream
 }
 
 template <typename T>
-bool HelicalInterpolation180<T>::calculate_on_cpu(const T*, T*) const
+bool HelicalInterpolation180<T>::calculate_on_cpu(const T* in, T* out) const
 {
-    return no_cpu("HelicalInterpolation180::calculate_on_cpu");
+    const auto& p = this->param_;
+    const size_t no = static_cast<size_t>(p.nz) * p.view_per_rot * p.ns;
+    Staging s;
+    T* d = s.in(out, no);
+    return staged(s, out, d, no, [&] {
+        return calculate_on_gpu(s.in(in, static_cast<size_t>(p.na) * p.nt * p.ns), d,
+                                s.stream());
+    });
 }
 
 template <typename T>

@@ -150,11 +150,13 @@ bool erase(int handle)
 
 extern "C" {
 
-bool set_device(const int device)
+// device < 0 is the reference's CPU mode (numpy_ext/common.cu:29-49): the
+// arrays are host memory either way, and this library computes them on GPU 0.
+bool set_device(const int device_in)
 {
     State& s = state();
     std::lock_guard<std::mutex> lk(s.mu);
-    if (device < 0) return false;  // no CPU mode in this library
+    const int device = device_in < 0 ? 0 : device_in;
     if (hipSetDevice(device) != hipSuccess) return false;
     if (s.stream && s.device != device) {
         (void)hipStreamDestroy(s.stream);

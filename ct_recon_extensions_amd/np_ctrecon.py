@@ -65,7 +65,8 @@ class FanFBP2DAngle:
         self._param = param
         itype = 1
         if not lib.set_device(param["device"]):
-            raise RuntimeError("set_device failed (this library needs a GPU, device >= 0)")
+            raise RuntimeError("set_device failed (this library computes on a GPU; device < 0 "
+                               "selects GPU 0)")
         self._angles = np.ascontiguousarray(param["angles"], dtype=np.double)
         self._fw_handle = lib.fan_weighting_create(param["ns"], param["na"], param["ds"],
                                                    param["offset_s"], param["dso"],

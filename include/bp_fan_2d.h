@@ -9,8 +9,10 @@
  * into device buffers.  Operator classes take device pointers in
  * calculate_on_gpu, enqueue on the stream and synchronise it before
  * returning (the reference's contract: results ready on return).  The
- * operators' calculate_on_cpu is not provided by this GPU library and
- * returns false (ctr_last_error() says why).
+ * operators' calculate_on_cpu takes host pointers, as the reference's does,
+ * and computes on the GPU: it stages the buffers through device memory on a
+ * stream of its own and runs the same kernels (same results as
+ * calculate_on_gpu).  Failures return false with ctr_last_error() set.
  */
 #ifndef CT_RECON_MI355X_BP_FAN_2D_H_
 #define CT_RECON_MI355X_BP_FAN_2D_H_

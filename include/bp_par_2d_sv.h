@@ -4,9 +4,9 @@
  * hipStream_t GPU entry points.  The operator writes every (pixel, view)
  * value, img[(iy*nx + ix)*na + ia], with no sum and no scale; the Prep fills
  * the parallel BP's tables (xcos[ix*na + ia], ysin[iy*na + ia]).
- * Conventions as in bp_fan_2d.h: calculate_on_cpu of the operator returns
- * false (no CPU implementation), calculate_on_gpu takes device pointers and
- * synchronises the stream before returning.
+ * Conventions as in bp_fan_2d.h: calculate_on_cpu of the operator takes
+ * host pointers (computed on the GPU), calculate_on_gpu takes device
+ * pointers and synchronises the stream before returning.
  */
 #ifndef CT_RECON_MI355X_BP_PAR_2D_SV_H_
 #define CT_RECON_MI355X_BP_PAR_2D_SV_H_

@@ -4,15 +4,17 @@
  * hipStream_t GPU entry point.  proj_in is the multi-row helical scan
  * [na][nt][ns] (view, detector row, channel), proj_out the nz fan-beam
  * sinograms [nz][view_per_rot][ns]; both are device pointers for
- * calculate_on_gpu, which synchronises the stream before it returns.
+ * calculate_on_gpu, which synchronises the stream before it returns, and
+ * host pointers for calculate_on_cpu.
  *
  * Errors follow the reference: a slice whose views the scan does not cover
  * throws std::runtime_error("View not enough for this slice!") before any
  * work, and a sample with no detector row in either window throws
  * std::runtime_error("No row data available!") (after the launch: the other
  * samples are written, that one is NaN).  A HIP failure returns false with
- * the message in ctr_last_error().  calculate_on_cpu is the reference's CPU
- * code, which this GPU library does not provide: it returns false.
+ * the message in ctr_last_error().  calculate_on_cpu takes host pointers, as
+ * the reference's does, and computes on the GPU (buffers staged through
+ * device memory), with the same results and errors.
  */
 #ifndef CT_RECON_MI355X_HELICAL_INTERPOLATION_H_
 #define CT_RECON_MI355X_HELICAL_INTERPOLATION_H_

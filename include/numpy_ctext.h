@@ -11,8 +11,8 @@
  * result back and returns when it is ready (in may equal out).
  *
  * Differences from the reference, all on its error paths:
- *   - set_device(d < 0) returns false: there is no CPU mode (the reference
- *     would run its CPU code);
+ *   - set_device(d < 0), the reference's CPU mode, computes on GPU 0 (the
+ *     arrays are host memory in both modes);
  *   - nothing throws across the C boundary: failures return false / 0;
  *   - fan_bp_2d_angle_create copies `angles` (the reference keeps the
  *     caller's pointer until the first run and leaks its device copy);

@@ -3,7 +3,8 @@
 // tests/test_class_api.py.  Usage:
 //   class_api_main cpu <dir>   tables from every Prep::calculate_on_cpu
 //   class_api_main gpu <dir>   every operator on the GPU, float and double
-//                              (helical rebinning included)
+//                              (helical rebinning included), then every
+//                              operator again through calculate_on_cpu
 // Inputs are read from <dir>/*.in.f64 and outputs written to <dir>/*.f64 as
 // raw little-endian doubles; geometries are fixed and mirrored in the test.
 #include <hip/hip_runtime.h>
@@ -121,9 +122,6 @@ static int cpu()
     std::vector<double> h(2 * kFilt.ns + 1);
     CHECK(RampFilterPrep(kFilt).calculate_on_cpu(h.data()));
     save("ramp_taps", h);
-    // operators have no CPU path in this library
-    std::vector<float> pf(kFan.na * kFan.ns), imf(kFan.nx * kFan.ny);
-    CHECK(!FanBackprojection2DPixDriven<float>(kFan).calculate_on_cpu(pf.data(), imf.data(), x.data(), y.data(), sc.data()));
     std::puts("cpu ok");
     return 0;
 }
@@ -225,6 +223,95 @@ static int gpu_typed(const char* suf, hipStream_t st)
     return 0;
 }
 
+// Every operator through calculate_on_cpu: host tables from the Prep
+// classes, host inputs and outputs; saved as <name>_host_<suffix> for a
+// bitwise comparison with the device runs above.
+template <typename T>
+static std::vector<T> host_in(const std::string& name, size_t n)
+{
+    const std::vector<double> d = load(name, n);
+    return std::vector<T>(d.begin(), d.end());
+}
+
+template <typename T>
+static int host_typed(const char* suf)
+{
+    const std::string s = std::string("_host_") + suf;
+    {
+        std::vector<double> x(kFan.nx), y(kFan.ny), sc(2 * kFan.na);
+        CHECK(FanBackprojection2DPixDrivenPrep(kFan).calculate_on_cpu(x.data(), y.data(), sc.data()));
+        std::vector<T> proj = host_in<T>("fan_proj", kFan.na * kFan.ns), img(kFan.nx * kFan.ny);
+        CHECK(FanBackprojection2DPixDriven<T>(kFan).calculate_on_cpu(proj.data(), img.data(), x.data(), y.data(), sc.data()));
+        save("fan_bp" + s, img);
+        std::vector<T> im2 = host_in<T>("fan_img", kFan.nx * kFan.ny), grad(kFan.na * kFan.ns);
+        CHECK(FanBackprojection2DPixDrivenGrad<T>(kFan).calculate_on_cpu(im2.data(), grad.data(), x.data(), y.data(), sc.data()));
+        save("fan_bpt" + s, grad);
+    }
+    {
+        const std::vector<double> ang = load("fana_angles", kFanA.na);
+        std::vector<double> x(kFanA.nx), y(kFanA.ny), sc(2 * kFanA.na);
+        CHECK(FanBackprojection2DAnglePixDrivenPrep(kFanA).calculate_on_cpu(ang.data(), x.data(), y.data(), sc.data()));
+        std::vector<T> proj = host_in<T>("fan_proj", kFanA.na * kFanA.ns), img(kFanA.nx * kFanA.ny);
+        CHECK(FanBackprojection2DAnglePixDriven<T>(kFanA).calculate_on_cpu(proj.data(), img.data(), x.data(), y.data(), sc.data()));
+        save("fana_bp" + s, img);
+    }
+    {
+        std::vector<double> xc(kPar.na * kPar.nx), ys(kPar.na * kPar.ny);
+        CHECK(ParallelBackprojection2DPixDrivenPrep(kPar).calculate_on_cpu(xc.data(), ys.data(), nullptr));
+        std::vector<T> proj = host_in<T>("par_proj", kPar.na * kPar.ns), img(kPar.nx * kPar.ny);
+        CHECK(ParallelBackprojection2DPixDriven<T>(kPar).calculate_on_cpu(proj.data(), img.data(), xc.data(), ys.data(), nullptr));
+        save("par_bp" + s, img);
+        std::vector<T> im2 = host_in<T>("par_img", kPar.nx * kPar.ny), grad(kPar.na * kPar.ns);
+        CHECK(ParallelBackprojection2DPixDrivenGrad<T>(kPar).calculate_on_cpu(im2.data(), grad.data(), nullptr, nullptr, nullptr));
+        save("par_bpt" + s, grad);
+        std::vector<T> sv(kSv.nx * kSv.ny * kSv.na);
+        CHECK(ParallelSingleViewBp2DPixDriven<T>(kSv).calculate_on_cpu(proj.data(), sv.data(), xc.data(), ys.data(), nullptr));
+        save("svbp" + s, sv);
+    }
+    {
+        std::vector<double> sc(2 * kFp.na), bo(2 * kFp.na * kFp.ns), w(kFp.na), pos(2 * kFp.na * kFp.nx);
+        std::vector<int> u(kFp.na), ug(kFp.na);
+        CHECK(ParallelProjection2DRayDrivenPrepare(kFp).calculate_on_cpu(sc.data(), bo.data(), u.data()));
+        CHECK(ParallelProjection2DRayDrivenGradPrep(kFp).calculate_on_cpu(w.data(), pos.data(), ug.data()));
+        std::vector<T> img = host_in<T>("fp_img", kFp.nx * kFp.ny), proj(kFp.na * kFp.ns);
+        CHECK(ParallelProjection2DRayDriven<T>(kFp).calculate_on_cpu(img.data(), proj.data(), sc.data(), bo.data(), u.data()));
+        save("fp" + s, proj);
+        std::vector<T> pr2 = host_in<T>("fp_proj", kFp.na * kFp.ns), img2(kFp.nx * kFp.ny);
+        CHECK(ParallelProjection2DRayDrivenGrad<T>(kFp).calculate_on_cpu(pr2.data(), img2.data(), w.data(), pos.data(), ug.data()));
+        save("fpt" + s, img2);
+        std::vector<double> bo3(3 * kFp.na * kFp.ns);
+        CHECK(ParallelProjection2DDisDrivenPrep(kFp).calculate_on_cpu(sc.data(), bo3.data(), u.data()));
+        CHECK(ParallelProjection2DDisDriven<T>(kFp).calculate_on_cpu(img.data(), proj.data(), sc.data(), bo3.data(), u.data()));
+        save("fpdd" + s, proj);
+        CHECK(ParallelProjection2DDisDrivenGradPrep(kFp).calculate_on_cpu(w.data(), pos.data(), ug.data()));
+        CHECK(ParallelProjection2DDisDrivenGrad<T>(kFp).calculate_on_cpu(pr2.data(), img2.data(), w.data(), pos.data(), ug.data()));
+        save("fpddg" + s, img2);
+        std::vector<double> rb(2 * kFpRc.na * kFpRc.ns);
+        std::vector<int> rn(kFpRc.na * kFpRc.ns);
+        CHECK(ParallelProjection2DRayCastingPrepare(kFpRc).calculate_on_cpu(sc.data(), rb.data(), rn.data()));
+        CHECK(ParallelProjection2DRayCasting<T>(kFpRc).calculate_on_cpu(img.data(), proj.data(), sc.data(), rb.data(), rn.data()));
+        save("fprc" + s, proj);
+    }
+    {
+        std::vector<double> h(2 * kFilt.ns + 1);
+        CHECK(RampFilterPrep(kFilt).calculate_on_cpu(h.data()));
+        std::vector<T> in = host_in<T>("ramp_in", kFilt.nrow * kFilt.ns), out(kFilt.nrow * kFilt.ns);
+        CHECK(RampFilter<T>(kFilt).calculate_on_cpu(in.data(), h.data(), out.data()));
+        save("ramp" + s, out);
+        CHECK(RampFilterGrad<T>(kFilt).calculate_on_cpu(in.data(), h.data(), out.data()));
+        save("rampg" + s, out);
+        std::vector<T> wi = host_in<T>("fw_in", kFw.nrow * kFw.ns), wo(kFw.nrow * kFw.ns);
+        CHECK(FanWeighting<T>(kFw).calculate_on_cpu(wi.data(), wo.data()));
+        save("fw" + s, wo);
+        CHECK(FanWeightingGrad<T>(kFw).calculate_on_cpu(wi.data(), wo.data()));
+        save("fwg" + s, wo);
+    }
+    // a null input is an error, reported as false
+    std::vector<T> o1(kFw.nrow * kFw.ns);
+    CHECK(!FanWeighting<T>(kFw).calculate_on_cpu(nullptr, o1.data()));
+    return 0;
+}
+
 // Helical rebinning; the geometry comes from <dir>/heli_geom.in.f64 (the
 // 18 fields in constructor order), the scan from heli_proj.in.f64.
 static HelicalInterpolationParam heli_geom(const std::vector<double>& v)
@@ -246,7 +333,14 @@ static int heli_typed(const char* suf, hipStream_t st)
     save("heli360_" + s, out.get());
     CHECK(HelicalInterpolation180<T>(p).calculate_on_gpu(in.p, out.p, st));
     save("heli180_" + s, out.get());
-    CHECK(!HelicalInterpolation360<T>(p).calculate_on_cpu(nullptr, nullptr));
+    {   // the same through host pointers
+        std::vector<T> hin = host_in<T>("heli_proj", static_cast<size_t>(p.na) * p.nt * p.ns);
+        std::vector<T> hout(static_cast<size_t>(p.nz) * p.view_per_rot * p.ns);
+        CHECK(HelicalInterpolation360<T>(p).calculate_on_cpu(hin.data(), hout.data()));
+        save("heli360_host_" + s, hout);
+        CHECK(HelicalInterpolation180<T>(p).calculate_on_cpu(hin.data(), hout.data()));
+        save("heli180_host_" + s, hout);
+    }
     // a slice one view too close to the scan start: the reference's exception
     HelicalInterpolationParam q = p;
     q.z0 = p.couch_begin + (p.view_per_rot - 1) * p.couch_mov;
@@ -266,6 +360,7 @@ static int gpu()
     CHECK(hipStreamCreate(&st) == hipSuccess);
     if (gpu_typed<float>("f32", st) || gpu_typed<double>("f64", st)) return 1;
     if (heli_typed<float>("f32", st) || heli_typed<double>("f64", st)) return 1;
+    if (host_typed<float>("f32") || host_typed<double>("f64")) return 1;
     CHECK(hipStreamDestroy(st) == hipSuccess);
     std::puts("gpu ok");
     return 0;

@@ -121,6 +121,12 @@ def test_class_api_operators_gpu(driver, tmp_path):
             _read(tmp_path, f"fpddg_{suf}"),
             O.fp_dd_grad(gq, inputs["fp_proj"].astype(dt),
                          O.fp_dd_grad_prep(gq)).ravel().astype(np.float64))
+        # calculate_on_cpu (host buffers, computed on the GPU): bitwise the
+        # device result
+        for name in ("fan_bp", "fan_bpt", "fana_bp", "par_bp", "par_bpt", "svbp", "fp", "fpt",
+                     "fpdd", "fpddg", "fprc", "ramp", "rampg", "fw", "fwg", "heli360", "heli180"):
+            np.testing.assert_array_equal(_read(tmp_path, f"{name}_host_{suf}"),
+                                          _read(tmp_path, f"{name}_{suf}"), err_msg=name)
         hh = O.heli(**HELI)
         for mode in (360, 180):
             want, code = O.helical(hh, inputs["heli_proj"].astype(dt), mode)
@@ -144,7 +150,6 @@ def _nplib():
 
 def test_numpy_abi_error_paths():
     lib = _nplib()
-    assert lib.set_device(-1) is False           # no CPU mode
     assert lib.fan_weighting_create(0, 4, 1.0, 0.0, 10.0, 20.0, 1) == 0
     assert lib.ramp_filter_create(8, 2, 1.0, 10.0, 7) == 0
     assert lib.fan_weighting_destroy(12345) is False
@@ -173,3 +178,6 @@ def test_numpy_fan_fbp_angle_matches_oracle():
     ref = O.fan_bp(g, O.ramp(g, O.fanw(g, proj), O.ramp_prep(g)),
                    O.fan_angle_prep(g, angles), scaled=False) * param["orbit"]
     assert rel_l2(recon, ref) < 1e-12
+    # device -1, the reference's CPU mode: the same result, computed on GPU 0
+    recon_cpu_mode = FanFBP2DAngle(dict(param, device=-1)).run(proj)
+    np.testing.assert_array_equal(recon_cpu_mode, recon)
