@@ -385,11 +385,16 @@ def bench_lsq(torch, steps):
         "rel_err_last": float((x - xt).norm() / xt.norm()),
         "fp_ray_samples_per_s": 8 * scan.na * scan.ns * scan.n / (ms_fp * 1e-3),
         "fpt_pixel_views_per_s": 8 * scan.na * scan.n ** 2 / (ms_fpt * 1e-3),
-        "roofline_fp": _valu_roofline("fp_lds_kernel", 8 * scan.na * scan.ns * scan.n,
-                                      FP_SLOTS_PER_SAMPLE, ms_fp, "ray sample"),
-        "roofline_fpt": _valu_roofline("bp_forward_kernel<FptGeo> + border passes",
-                                       8 * scan.na * scan.n ** 2, FPT_SLOTS_PER_PV, ms_fpt,
-                                       "pixel*view"),
+        "roofline_fp": _valu_roofline(
+            "fp_lds_kernel", 8 * scan.na * scan.ns * scan.n,
+            FP_GEOM_SLOTS / FP_SLICES_PER_WG + FP_SLICE_SLOTS, ms_fp,
+            f"ray sample ({FP_GEOM_SLOTS} per ray-row shared by {FP_SLICES_PER_WG} slices "
+            f"+ {FP_SLICE_SLOTS} per slice)"),
+        "roofline_fpt": _valu_roofline(
+            "bp_forward_kernel<FptGeo> + border passes", 8 * scan.na * scan.n ** 2,
+            FPT_GEOM_SLOTS / FPT_SLICES_PER_WG + FPT_SLICE_SLOTS, ms_fpt,
+            f"pixel*view*slice ({FPT_GEOM_SLOTS} per pixel*view shared by "
+            f"{FPT_SLICES_PER_WG} slices + {FPT_SLICE_SLOTS} per slice)"),
     }
 
 
@@ -398,7 +403,7 @@ def _valu_roofline(kernel, units, slots_per_unit, ms, unit_name):
     return {"kernel": kernel, "bound": "valu", "achieved": round(achieved / 1e12, 3),
             "peak": round(PEAK_VALU_SLOTS / 1e12, 3), "unit": "Tslot/s (FP32 VALU lane-issues)",
             "frac": round(achieved / PEAK_VALU_SLOTS, 4),
-            "model": f"{slots_per_unit} slots per {unit_name} x {units:.4g} per launch"}
+            "model": f"{slots_per_unit:g} slots per {unit_name} x {units:.4g} per launch"}
 
 
 def bench_ragged(torch, reps):
@@ -472,12 +477,17 @@ def bench_filter_precision(torch, reps):
 
 
 BWD_SEED = 20261209
-# Algorithmic VALU issue slots of the ray-driven FP and of its transpose
-# (per ray sample and per pixel*view): 7 and 12, counting the interpolation,
-# the position step and the accumulation of one sample, as the per-unit
-# model of these two kernels.
-FP_SLOTS_PER_SAMPLE = 7
-FPT_SLOTS_PER_PV = 12
+# Algorithmic VALU issue slots of the ray-driven FP and of its transpose,
+# split like the fan BP's into geometry shared by the slices one workgroup
+# carries and work per slice.
+# FP, per ray and image row: the FP64 position step and the inclusion test
+# (2 + 2, FP64 at half rate), the tap index and fraction (2) = 6 per ray-row,
+# shared by the 4 slices of a workgroup; per slice the lerp into the sum, 2.
+# FP^T on the engine, per pixel and view: the channel position (a linear
+# form, 2), its floor and fraction (2), the two tap weights and the edge
+# rule (4) = 8, shared by the 8 slices of a workgroup; per slice 2.
+FP_GEOM_SLOTS, FP_SLICE_SLOTS, FP_SLICES_PER_WG = 6, 2, 4
+FPT_GEOM_SLOTS, FPT_SLICE_SLOTS, FPT_SLICES_PER_WG = 8, 2, 8
 PEAK_MFMA_BF16_TFLOPS = 2500.0
 PEAK_FP32_MFMA_TFLOPS = 157.3
 

@@ -28,6 +28,7 @@
 // (window means, the z interpolation; for 180LI the conjugate window) bound
 // the kernel ahead of its HBM reads.
 #include "ctr_common.hpp"
+#include "tile_common.hpp"
 
 #include <cmath>
 
@@ -88,8 +89,12 @@ __global__ __launch_bounds__(kHeliTX* kHeliTY) void helical_kernel(
     const double* __restrict__ sscor, const double* __restrict__ slices,
     const T* __restrict__ in, T* __restrict__ out, unsigned int* status)
 {
-    const int bz = blockIdx.x / P.nvx;  // batch * nz + slice
-    const int iv = (blockIdx.x - bz * P.nvx) * kHeliTY + threadIdx.y;
+    // Rows in (batch, slice, view) order, every XCD given a contiguous run of
+    // them: neighbouring output views of a slice read neighbouring views (and,
+    // for 180LI, conjugate lines shifted by one view), so they share L2.
+    const int l = xcd_tile(blockIdx.x, gridDim.x);
+    const int bz = l / P.nvx;  // batch * nz + slice
+    const int iv = (l - bz * P.nvx) * kHeliTY + threadIdx.y;
     if (iv >= P.vpr) return;
     const int b = bz / P.nz, iz = bz - b * P.nz;
     const double z = slices[4 * iz];
