@@ -9,17 +9,17 @@ caller's sinogram is not overwritten (the reference weights it in place).
 """
 from __future__ import annotations
 
+import ctypes as C
 import os
 
 import numpy as np
 import numpy.ctypeslib as npct
 
-_array1dd = npct.ndpointer(np.double, 1, flags="C")
-_array2dd = npct.ndpointer(np.double, 2, flags="C")
-_uint = npct.ctypes.c_uint
-_int = npct.ctypes.c_int
-_double = npct.ctypes.c_double
-_bool = npct.ctypes.c_bool
+# argument kinds of the eleven entry points: contiguous float64 vectors and
+# matrices, and the C scalars of include/numpy_ctext.h
+_VEC = npct.ndpointer(np.float64, ndim=1, flags="C_CONTIGUOUS")
+_MAT = npct.ndpointer(np.float64, ndim=2, flags="C_CONTIGUOUS")
+_U, _I, _D, _B = C.c_uint, C.c_int, C.c_double, C.c_bool
 
 _LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 _lib = None
@@ -30,20 +30,17 @@ def _load():
     if _lib is None:
         lib = npct.load_library("libnumpy_ctext.so", _LIBDIR)
         sigs = {
-            "set_device": (_bool, [_int]),
-            "clear": (_bool, []),
-            "fan_weighting_create": (_int, [_uint, _uint, _double, _double, _double, _double,
-                                            _int]),
-            "fan_weighting_run": (_bool, [_int, _array2dd, _array2dd]),
-            "fan_weighting_destroy": (_bool, [_int]),
-            "ramp_filter_create": (_int, [_uint, _uint, _double, _double, _int]),
-            "ramp_filter_run": (_bool, [_int, _array2dd, _array2dd]),
-            "ramp_filter_destroy": (_bool, [_int]),
-            "fan_bp_2d_angle_create": (_int, [_array1dd, _uint, _uint, _double, _double, _uint,
-                                              _uint, _double, _double, _double, _double,
-                                              _double, _double, _double]),
-            "fan_bp_2d_angle_run": (_bool, [_int, _array2dd, _array2dd]),
-            "fan_bp_2d_angle_destroy": (_bool, [_int]),
+            "set_device": (_B, [_I]),
+            "clear": (_B, []),
+            "fan_weighting_create": (_I, [_U, _U, _D, _D, _D, _D, _I]),
+            "fan_weighting_run": (_B, [_I, _MAT, _MAT]),
+            "fan_weighting_destroy": (_B, [_I]),
+            "ramp_filter_create": (_I, [_U, _U, _D, _D, _I]),
+            "ramp_filter_run": (_B, [_I, _MAT, _MAT]),
+            "ramp_filter_destroy": (_B, [_I]),
+            "fan_bp_2d_angle_create": (_I, [_VEC, _U, _U, _D, _D, _U, _U] + [_D] * 7),
+            "fan_bp_2d_angle_run": (_B, [_I, _MAT, _MAT]),
+            "fan_bp_2d_angle_destroy": (_B, [_I]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -93,15 +90,18 @@ class FanFBP2DAngle:
         lib = _lib
         proj = np.array(proj, dtype=np.double, order="C", copy=True)
         na, ns = proj.shape
-        if na != self._param["na"] or ns != self._param["ns"]:
-            raise RuntimeError("Input shape not available!")
+        if (na, ns) != (self._param["na"], self._param["ns"]):
+            raise RuntimeError(f"sinogram is {na} x {ns}; this operator takes "
+                               f"{self._param['na']} x {self._param['ns']} (views x channels)")
         weighted = np.empty_like(proj)
         filtered = np.empty_like(proj)
         recon = np.zeros([self._param["ny"], self._param["nx"]], dtype=np.double)
-        if not lib.fan_weighting_run(self._fw_handle, proj, weighted):
-            raise RuntimeError("Fan weighting failed!")
-        if not lib.ramp_filter_run(self._flt_handle, weighted, filtered):
-            raise RuntimeError("Ramp filter failed!")
-        if not lib.fan_bp_2d_angle_run(self._bp_handle, filtered, recon):
-            raise RuntimeError("Backprojection failed!")
+        for stage, ok in (("fan weighting", lambda: lib.fan_weighting_run(self._fw_handle, proj,
+                                                                          weighted)),
+                          ("ramp filter", lambda: lib.ramp_filter_run(self._flt_handle, weighted,
+                                                                      filtered)),
+                          ("backprojection", lambda: lib.fan_bp_2d_angle_run(
+                              self._bp_handle, filtered, recon))):
+            if not ok():
+                raise RuntimeError(f"FanFBP2DAngle: the {stage} stage failed on the GPU")
         return recon * self._param["orbit"]
