@@ -462,7 +462,7 @@ def bench_filter_precision(torch, reps):
     # Parity-split Toeplitz GEMM: M = rows, N = K = ns, half the dense MACs.
     macs = nb * na * ns * ns / 2
     hbm = 2 * x.numel() * 4
-    for impl, kern, mult, peak in (("bf16x3", "ramp_bf16x3_kernel", 3, PEAK_MFMA_BF16_TFLOPS),
+    for impl, kern, mult, peak in (("bf16x3", "ramp_persist_bf16x3_kernel<8, 12, 48>", 3, PEAK_MFMA_BF16_TFLOPS),
                                    ("mfma", "ramp_mfma_kernel", 1, PEAK_FP32_MFMA_TFLOPS)):
         tf = mult * 2 * macs / (res["ms"][impl] * 1e9)
         res["roofline_" + ("fp32_mfma" if impl == "mfma" else impl)] = {
