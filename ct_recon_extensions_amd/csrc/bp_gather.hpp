@@ -4,25 +4,29 @@
 // s = s(pixel, view) and the taps floor(s), floor(s)+1.  Its transpose
 // therefore gives channel c of a view the sum of img * w * (1 - frac) over
 // the pixels with floor(s) = c, plus img * w * frac over those with
-// floor(s) = c - 1: the pixels whose s lies in (c-1, c+1).  Here one lane
-// owns one (view, channel) output and adds those contributions in registers,
-// in a fixed order (image rows in increasing order, pixels along a row in
-// increasing order): no atomics, no cleared output, and the result is
-// bitwise repeatable.
+// floor(s) = c - 1.  Here one lane owns one (view, channel interval
+// [i, i+1)): it finds the pixels with floor(s) = i and sums, in registers,
+// both P_i = sum img * w * (1 - frac) and Q_i = sum img * w * frac; channel c
+// is then P_c + Q_{c-1}, Q taken from the lane on the left.  Every pixel is
+// a candidate of one lane (two, when the channel on the left is wanted too)
+// and all sums run in a fixed order (image rows in increasing order, pixels
+// along a row in increasing order): no atomics, no cleared output, and the
+// result is bitwise repeatable.
 //
 // Finding the pixels: the level line s = s* of both geometries is a
 // straight line in pixel-index space (a ray: r - tan(gamma*) d = 0 for the
 // fan, x cos + y sin = const for parallel beams), written L(ix, iy) =
 // pi ix + qj iy + rr with L >= 0 exactly where s >= s* (Geo::boundary).  On
-// one image row the pixels with s in [c-1-eps, c+1+eps] are then the
+// one image row the pixels with s in [i-eps, i+1+eps] are then the
 // intersection of two half-lines, computed per row from two linear bounds;
 // each candidate is evaluated with the forward kernel's own per-tile
 // anchors and FP32 pixel geometry (Geo::pixel), so the weights are those of
 // the forward.  The margin eps (1/32 channel) keeps every pixel of the
 // support inside the candidate interval despite FP32 rounding of the bounds.
 //
-// Work layout (gfx950): a workgroup = 4 views (one wave each) x 64 channels
-// (one lane each) x NB slices.  It walks the image in bands of R rows; per
+// Work layout (gfx950): a workgroup = 4 views (one wave each) x 64 channel
+// intervals (one lane each: 63 output channels, lane 0 only supplies the Q
+// of the interval left of them) x NB slices.  It walks the image in bands of R rows; per
 // band the union of its lanes' candidate intervals is staged into LDS in
 // chunks of WMAX pixels (slice-interleaved: [row][slice group][pixel][4
 // slices], one ds_read_b128 per 4 slices), together with the anchors of the
@@ -42,7 +46,8 @@ namespace ctr {
 
 constexpr int kGatherTile = 16;     // anchors of 16 x 16 tiles (the forward's default)
 constexpr int kGatherViews = 4;     // views per workgroup (one wave each)
-constexpr int kGatherLanes = 64;    // channels per workgroup (one lane each)
+constexpr int kGatherLanes = 64;    // channel intervals per workgroup (one lane each)
+constexpr int kGatherOut = kGatherLanes - 1;  // output channels per workgroup
 constexpr double kGatherEps = 1.0 / 32;
 
 // anchors[view][tile] of 16 x 16 tiles: the FP64 per-(tile, view) geometry,
@@ -157,17 +162,18 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
     typename Geo::Problem Pg = P;
     if constexpr (HasPin<Geo>::value) Geo::pin(Pg);
     const int tid = threadIdx.x, lane = tid & 63, wave = uniform_i(tid >> 6);
-    const int c0 = blockIdx.x * kGatherLanes, v0 = blockIdx.y * V, b0 = blockIdx.z * NB;
-    const int c = c0 + lane, view = v0 + wave;
+    // lane: interval [ci, ci + 1); lanes 1..63 write channels c0 .. c0 + 62
+    const int c0 = blockIdx.x * kGatherOut, v0 = blockIdx.y * V, b0 = blockIdx.z * NB;
+    const int ci = c0 - 1 + lane, view = v0 + wave;
     const int nb = min(NB, L.nbatch - b0);
-    const bool active = c < P.ns && view < P.na;
+    const bool active = ci < P.ns && view < P.na;
 
     // Frame of the workgroup, from the level line through its middle view
     // and channel: walk rows (0) when that line is closer to the columns.
     int frame;
     {
         const int vm = v0 + (min(V, P.na - v0) - 1) / 2;
-        const int cm = c0 + (min(kGatherLanes, P.ns - c0) - 1) / 2;
+        const int cm = c0 + (min(kGatherOut, P.ns - c0) - 1) / 2;
         double pi, qj, rr;
         Geo::boundary(P, L.T, vm, static_cast<double>(cm), pi, qj, rr);
         frame = uniform_i(fabs(pi) >= fabs(qj) ? 0 : 1);
@@ -176,11 +182,11 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
     const float* src = frame ? L.imgT : L.img;
     const unsigned int slice_bytes = static_cast<unsigned int>(nu) * nw * 4u;
 
-    // This lane's two constraints: s >= c - 1 - eps and s <= c + 1 + eps.
+    // This lane's two constraints: s >= ci - eps and s <= ci + 1 + eps.
     GatherBound bd[2];
     {
         const int vv = active ? view : min(view, P.na - 1);
-        const double sl[2] = {c - 1 - kGatherEps, c + 1 + kGatherEps};
+        const double sl[2] = {ci - kGatherEps, ci + 1 + kGatherEps};
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             double pi, qj, rr;
@@ -190,9 +196,9 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
         }
     }
 
-    float acc[NB];
+    float accp[NB], accq[NB];  // P and Q of this lane's interval
 #pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = 0.0f;
+    for (int b = 0; b < NB; ++b) accp[b] = accq[b] = 0.0f;
 
     const int nbands = (nu + R - 1) / R;
     for (int band = 0; band < nbands; ++band) {
@@ -384,29 +390,37 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
                 // the forward's edge rule, only for tiles that reach past the
                 // detector (few, and the same for most lanes of a wave)
                 if (!A.inside) wt = on_detector(s, fr, P.ns) ? wt : 0.0f;
-                const float w1 = wt * fr, w0 = wt - w1;
-                const float wc = s == c ? w0 : (s + 1 == c ? w1 : 0.0f);
+                const float wm = s == ci ? wt : 0.0f;
+                const float w1 = wm * fr, w0 = wm - w1;
                 if constexpr (S::kQuad) {
 #pragma unroll
                     for (int g = 0; g < NG; ++g) {
-                        acc[4 * g] = fmaf(wc, pv[g].x, acc[4 * g]);
-                        acc[4 * g + 1] = fmaf(wc, pv[g].y, acc[4 * g + 1]);
-                        acc[4 * g + 2] = fmaf(wc, pv[g].z, acc[4 * g + 2]);
-                        acc[4 * g + 3] = fmaf(wc, pv[g].w, acc[4 * g + 3]);
+                        accp[4 * g] = fmaf(w0, pv[g].x, accp[4 * g]);
+                        accp[4 * g + 1] = fmaf(w0, pv[g].y, accp[4 * g + 1]);
+                        accp[4 * g + 2] = fmaf(w0, pv[g].z, accp[4 * g + 2]);
+                        accp[4 * g + 3] = fmaf(w0, pv[g].w, accp[4 * g + 3]);
+                        accq[4 * g] = fmaf(w1, pv[g].x, accq[4 * g]);
+                        accq[4 * g + 1] = fmaf(w1, pv[g].y, accq[4 * g + 1]);
+                        accq[4 * g + 2] = fmaf(w1, pv[g].z, accq[4 * g + 2]);
+                        accq[4 * g + 3] = fmaf(w1, pv[g].w, accq[4 * g + 3]);
                     }
                 } else {
-                    acc[0] = fmaf(wc, p1, acc[0]);
+                    accp[0] = fmaf(w0, p1, accp[0]);
+                    accq[0] = fmaf(w1, p1, accq[0]);
                 }
             }
         }
     }
 
-    if (active) {
+    // channel ci = P_ci + Q_{ci-1}, the latter from the lane on the left
+    // (every lane takes part in the exchange)
+    const bool out = lane > 0 && ci >= 0 && active;
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
-            if (b < nb)
-                L.proj[(static_cast<size_t>(b0 + b) * P.na + view) * P.ns + c] =
-                    static_cast<float>(static_cast<double>(acc[b]) * L.scale);
+    for (int b = 0; b < NB; ++b) {
+        const float ql = __shfl_up(accq[b], 1);
+        if (out && b < nb)
+            L.proj[(static_cast<size_t>(b0 + b) * P.na + view) * P.ns + ci] =
+                static_cast<float>(static_cast<double>(accp[b] + ql) * L.scale);
     }
 }
 
@@ -431,7 +445,7 @@ int launch_gather_nb(GatherLaunch<Geo> L, hipStream_t st)
         CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(S::kLds)));
-    dim3 grid((L.P.ns + kGatherLanes - 1) / kGatherLanes, (L.P.na + kGatherViews - 1) / kGatherViews,
+    dim3 grid((L.P.ns + kGatherOut - 1) / kGatherOut, (L.P.na + kGatherViews - 1) / kGatherViews,
               (L.nbatch + NB - 1) / NB);
     hipLaunchKernelGGL(kern, grid, dim3(256), S::kLds, st, L);
     return check_launch("bp_gather_t_kernel");
