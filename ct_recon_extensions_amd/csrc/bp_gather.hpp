@@ -38,6 +38,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "bp_engine.hpp"
 #include "ctr_common.hpp"
 #include "tile_common.hpp"
@@ -134,8 +136,8 @@ struct GatherShape {
     // rows then fall on different LDS banks
     static constexpr int kRowF4 = (kQuad ? kGroups * WMAX : WMAX / 4) + 1;
     static constexpr int kImgFloats = R * kRowF4 * 4;
-    static constexpr int kAnchorDwords =
-        kGatherViews * kTiles * static_cast<int>(sizeof(typename Geo::Anchor) / 4);
+    // anchors are staged as their first 32 bytes (all the gather reads)
+    static constexpr int kAnchorDwords = kGatherViews * kTiles * 8;
     static constexpr size_t kLds = kImgFloats * sizeof(float) + kAnchorDwords * 4 + 64;
     static_assert(NB == 1 || kQuad, "slices per workgroup: 1 or a multiple of 4");
     static_assert(WMAX % (kQuad ? 16 : 64) == 0, "chunk width: whole DMA rows");
@@ -151,12 +153,14 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
     using S = GatherShape<Geo, NB, R, WMAX>;
     using Anchor = typename Geo::Anchor;
     constexpr int V = kGatherViews, K = S::kTiles, NG = S::kGroups;
-    constexpr int DW = static_cast<int>(sizeof(Anchor) / 4);
+    static_assert(sizeof(Anchor) % 16 == 0 && sizeof(Anchor) >= 32 &&
+                      offsetof(Anchor, inside) < 32 && offsetof(Anchor, is0) < 32,
+                  "the gather reads an anchor's first 32 bytes");
+    static_assert(kGatherViews * S::kTiles <= 256, "one thread per staged anchor");
     extern __shared__ __align__(16) unsigned char smem[];
     float* simg = reinterpret_cast<float*>(smem);
     unsigned int* sanc = reinterpret_cast<unsigned int*>(smem + S::kImgFloats * sizeof(float));
     int* sred = reinterpret_cast<int*>(smem + S::kImgFloats * sizeof(float) + S::kAnchorDwords * 4);
-    const Anchor* anch = reinterpret_cast<const Anchor*>(sanc);
 
     const auto& P = L.P;
     typename Geo::Problem Pg = P;
@@ -182,19 +186,52 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
     const float* src = frame ? L.imgT : L.img;
     const unsigned int slice_bytes = static_cast<unsigned int>(nu) * nw * 4u;
 
-    // This lane's two constraints: s >= ci - eps and s <= ci + 1 + eps.
-    GatherBound bd[2];
+    // This lane's two constraints, s >= ci - eps and s <= ci + 1 + eps, as
+    // one lower and one upper line along the row: w >= la + lb u and
+    // w <= ua + ub u.  A constraint parallel to the rows bounds nothing, and
+    // should both be lower (or upper) lines, one of them is kept: either way
+    // the candidates are a superset (the exact test in the loop decides).
+    double la = -1e30, lb = 0.0, ua = 1e30, ub = 0.0;
+    bool open = false;    // a side without a bound
+    bool lower0 = false;  // the first constraint is the lower line
     {
         const int vv = active ? view : min(view, P.na - 1);
         const double sl[2] = {ci - kGatherEps, ci + 1 + kGatherEps};
+        bool has_l = false, has_u = false;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             double pi, qj, rr;
             Geo::boundary(P, L.T, vv, sl[k], pi, qj, rr);
             const double cw = frame ? qj : pi, cu = frame ? pi : qj;
-            bd[k] = gather_bound(cw, cu, rr, k == 0 ? 1.0 : -1.0);
+            const GatherBound b = gather_bound(cw, cu, rr, k == 0 ? 1.0 : -1.0);
+            if (k == 0) lower0 = b.lower;
+            if (b.none) continue;
+            if (b.lower && !has_l) {
+                la = b.A;
+                lb = b.B;
+                has_l = true;
+            } else if (!b.lower && !has_u) {
+                ua = b.A;
+                ub = b.B;
+                has_u = true;
+            }
         }
+        open = active && !(has_l && has_u);
     }
+    // A lane's level lines turn monotonically with its channel (by less
+    // than pi over the wave), so while no line of the wave has turned
+    // through the row direction -- every lane has both bounds, all with the
+    // same orientation -- the bounds along a row move monotonically with the
+    // lane and the union of the wave's intervals is the hull of its first
+    // and last lanes'.  Otherwise every lane takes part in a reduction.
+    const bool hull = __ballot(open) == 0 && (__ballot(active && lower0) == 0 ||
+                                              __ballot(active && !lower0) == 0);
+    const int last = min(kGatherLanes - 1, P.ns - c0);  // last lane with ci < ns
+    // the image rows of the frame, per 4-slice group
+    const float* gsrc[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        gsrc[g] = src + static_cast<size_t>(min(b0 + 4 * g, L.nbatch - 1)) * nu * nw;
 
     float accp[NB], accq[NB];  // P and Q of this lane's interval
 #pragma unroll
@@ -205,45 +242,43 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
         const int u0 = band * R;
         const int rmax = min(R, nu - u0);
         // Bounds re-anchored at the band's first row (FP64), then FP32 per row.
-        float fa[2], fb[2];
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            fa[k] = static_cast<float>(bd[k].A + bd[k].B * u0);
-            fb[k] = static_cast<float>(bd[k].B);
-        }
-        // This lane's candidate range over the band's rows (a superset):
-        // lower bounds are linear in the row, so their smallest value is at
-        // an end row; likewise the largest upper bound.
+        const float fla = static_cast<float>(la + lb * u0), flb = static_cast<float>(lb);
+        const float fua = static_cast<float>(ua + ub * u0), fub = static_cast<float>(ub);
+        // This lane's candidate range over the band's rows (a superset): the
+        // bounds are linear in the row, so their extremes are at end rows.
         float lo = 3.0e38f, hi = -3.0e38f;
         if (active) {
-            float lmin = 3.0e38f, umax = -3.0e38f;
-            bool has_l = false, has_u = false;
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                if (bd[k].none) continue;
-                const float e0 = fa[k], e1 = fmaf(fb[k], static_cast<float>(rmax - 1), fa[k]);
-                if (bd[k].lower) {
-                    lmin = fminf(lmin, fminf(e0, e1));
-                    has_l = true;
-                } else {
-                    umax = fmaxf(umax, fmaxf(e0, e1));
-                    has_u = true;
-                }
+            const float rl = static_cast<float>(rmax - 1);
+            lo = fminf(fla, fmaf(flb, rl, fla));
+            hi = fmaxf(fua, fmaf(fub, rl, fua));
+        }
+        int ilo, ihi;
+        if (hull) {
+            // one pixel of margin on each side covers the rounding of the
+            // lanes' FP32 bounds about the hull
+            const float l0 = __shfl(lo, 0), l1 = __shfl(lo, last);
+            const float h0 = __shfl(hi, 0), h1 = __shfl(hi, last);
+            ilo = static_cast<int>(floorf(fmaxf(fminf(l0, l1), -2.0f))) - 1;
+            ihi = static_cast<int>(ceilf(fminf(fmaxf(h0, h1), static_cast<float>(nw + 1)))) + 1;
+            ilo = max(ilo, 0);
+            ihi = min(ihi, nw - 1);
+            if (!active || ilo > ihi) {
+                ilo = 1 << 30;
+                ihi = -(1 << 30);
             }
-            lo = has_l ? lmin : 0.0f;
-            hi = has_u ? umax : static_cast<float>(nw - 1);
-        }
-        int ilo = static_cast<int>(floorf(fmaxf(lo, -1.0f)));
-        int ihi = static_cast<int>(ceilf(fminf(hi, static_cast<float>(nw))));
-        ilo = max(ilo, 0);
-        ihi = min(ihi, nw - 1);
-        if (ilo > ihi) {
-            ilo = 1 << 30;
-            ihi = -(1 << 30);
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            ilo = min(ilo, __shfl_xor(ilo, o));
-            ihi = max(ihi, __shfl_xor(ihi, o));
+        } else {
+            ilo = static_cast<int>(floorf(fmaxf(lo, -1.0f)));
+            ihi = static_cast<int>(ceilf(fminf(hi, static_cast<float>(nw))));
+            ilo = max(ilo, 0);
+            ihi = min(ihi, nw - 1);
+            if (ilo > ihi) {
+                ilo = 1 << 30;
+                ihi = -(1 << 30);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                ilo = min(ilo, __shfl_xor(ilo, o));
+                ihi = max(ihi, __shfl_xor(ihi, o));
+            }
         }
         // previous band's candidates are done with the LDS before this
         // barrier; the reduction slots alternate between bands
@@ -276,8 +311,7 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
                     for (int g = 0; g < NG; ++g) {
                         const int ng = nb - 4 * g;
                         if (ng <= 0) break;
-                        const float* base =
-                            src + (static_cast<size_t>(b0 + 4 * g) * nu + u0 + r) * nw;
+                        const float* base = gsrc[g] + static_cast<size_t>(u0 + r) * nw;
                         const unsigned int range = ng >= 4 ? 4u * slice_bytes : ng * slice_bytes;
                         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                             const_cast<float*>(base), 0, range, 0x00020000);
@@ -308,22 +342,21 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
                 }
             }
             // ---- the anchors of the chunk's tiles, for the workgroup's views
-            {
-                const int ntiles = L.tiles_x * L.tiles_y;
-                const int tu = u0 / kGatherTile, tw0 = cw / kGatherTile;
+            if (tid < V * K) {
+                const int v = tid / K, k = tid - v * K;
+                const int tu = u0 / kGatherTile, tw = cw / kGatherTile + k;
                 const int tw_max = frame ? L.tiles_y : L.tiles_x;
-                const unsigned int* tab = reinterpret_cast<const unsigned int*>(L.anchors);
-                for (int d = tid; d < S::kAnchorDwords; d += 256) {
-                    const int a = d / DW, word = d - a * DW;
-                    const int v = a / K, k = a - v * K;
-                    const int tw = tw0 + k;
-                    unsigned int val = 0u;
-                    if (v0 + v < P.na && tw < tw_max) {
-                        const int tile = frame ? tw * L.tiles_x + tu : tu * L.tiles_x + tw;
-                        val = tab[(static_cast<size_t>(v0 + v) * ntiles + tile) * DW + word];
-                    }
-                    sanc[d] = val;
+                uint4 q0 = make_uint4(0u, 0u, 0u, 0u), q1 = q0;
+                if (v0 + v < P.na && tw < tw_max) {
+                    const int tile = frame ? tw * L.tiles_x + tu : tu * L.tiles_x + tw;
+                    const uint4* a = reinterpret_cast<const uint4*>(
+                        L.anchors + static_cast<size_t>(v0 + v) * L.tiles_x * L.tiles_y + tile);
+                    q0 = a[0];
+                    q1 = a[1];
                 }
+                uint4* d = reinterpret_cast<uint4*>(sanc) + 2 * tid;
+                d[0] = q0;
+                d[1] = q1;
             }
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
@@ -332,37 +365,32 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
             // counted loop over all of them (candidate k -> row r by the
             // prefix counts), so lanes with different rows do not diverge
             const int wlo = max(cw, 0), whi = min(cw + WMAX - 1, nw - 1);
-            int wst[R], pre[R + 1];
+            int dlt[R], pre[R + 1];  // row r: candidates pre[r] .. pre[r+1]-1 at w = kc + dlt[r]
             pre[0] = 0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float lb = -3.0e38f, ub = 3.0e38f;
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    if (bd[k].none) continue;
-                    const float e = fmaf(fb[k], static_cast<float>(r), fa[k]);
-                    if (bd[k].lower)
-                        lb = fmaxf(lb, e);
-                    else
-                        ub = fminf(ub, e);
-                }
-                const int w0 = max(static_cast<int>(ceilf(fmaxf(lb, -2.0f))), wlo);
-                const int w1 = min(static_cast<int>(floorf(fminf(ub, static_cast<float>(nw + 1)))), whi);
+                const float lbr = fmaf(flb, static_cast<float>(r), fla);
+                const float ubr = fmaf(fub, static_cast<float>(r), fua);
+                const int w0 = max(static_cast<int>(ceilf(fmaxf(lbr, -2.0f))), wlo);
+                const int w1 = min(static_cast<int>(floorf(fminf(ubr, static_cast<float>(nw + 1)))), whi);
                 const int n = (active && r < rmax) ? max(w1 - w0 + 1, 0) : 0;
-                wst[r] = w0;
+                dlt[r] = w0 - pre[r];
                 pre[r + 1] = pre[r] + n;
             }
             const int total = pre[R];
+            // the band's tile centre across the rows (bands do not straddle
+            // a tile row) and along them
+            const int uc = min((u0 & ~(kGatherTile - 1)) + kGatherTile / 2, (frame ? P.nx : P.ny) - 1);
+            const int wcl = (frame ? P.ny : P.nx) - 1;
             for (int kc = 0; kc < total; ++kc) {
-                int r = 0, base = 0, w = wst[0];
+                int r = 0, d = dlt[0];
 #pragma unroll
                 for (int q = 1; q < R; ++q) {
                     const bool past = kc >= pre[q];
                     r = past ? q : r;
-                    base = past ? pre[q] : base;
-                    w = past ? wst[q] : w;
+                    d = past ? dlt[q] : d;
                 }
-                w += kc - base;
+                const int w = kc + d;
                 // the candidate's image values first: their LDS reads are in
                 // flight while the geometry below is computed
                 float4 pv[NG];
@@ -375,16 +403,13 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
                 } else {
                     p1 = simg[r * S::kRowF4 * 4 + (w - cw)];
                 }
-                const int u = u0 + r;
-                const int ix = frame ? u : w, iy = frame ? w : u;
-                const Anchor A = *reinterpret_cast<const Anchor*>(
-                    reinterpret_cast<const unsigned char*>(anch) +
-                    static_cast<unsigned int>(wave * K + ((w - cw) >> 4)) *
-                        static_cast<unsigned int>(sizeof(Anchor)));
-                const int ic = min((ix & ~(kGatherTile - 1)) + kGatherTile / 2, P.nx - 1);
-                const int jc = min((iy & ~(kGatherTile - 1)) + kGatherTile / 2, P.ny - 1);
+                Anchor A{};
+                __builtin_memcpy(&A, sanc + 8 * (wave * K + ((w - cw) >> 4)), 32);
+                const float fu = static_cast<float>(u0 + r - uc);
+                const float fw = static_cast<float>(
+                    w - min((w & ~(kGatherTile - 1)) + kGatherTile / 2, wcl));
                 float f, wt;
-                Geo::pixel(Pg, A, static_cast<float>(ix - ic), static_cast<float>(iy - jc), f, wt);
+                Geo::pixel(Pg, A, frame ? fu : fw, frame ? fw : fu, f, wt);
                 const float fr = __builtin_amdgcn_fractf(f);
                 const int s = A.is0 + static_cast<int>(f);
                 // the forward's edge rule, only for tiles that reach past the
