@@ -153,16 +153,35 @@ __global__ __launch_bounds__(kHeliTX* kHeliTY) void helical_kernel(
                 const int sl = static_cast<int>(floor(ss_cor));
                 const int sr = static_cast<int>(ceil(ss_cor));
                 const double v = ss_cor - sl;
-                const T* pl = src + (static_cast<size_t>(vl) * P.nt + it0) * P.ns;
-                const T* pr = src + (static_cast<size_t>(vr) * P.nt + it0) * P.ns;
                 double left_value = 0, right_value = 0;
                 int count = 0;
-                for (int it = it0; it < it1; ++it, pl += P.ns, pr += P.ns) {
-                    left_value += (1 - v) * static_cast<double>(pl[sl]) +
-                                  v * static_cast<double>(pl[sr]);
-                    right_value += (1 - v) * static_cast<double>(pr[sl]) +
-                                   v * static_cast<double>(pr[sr]);
-                    ++count;
+                if (P.ns >= 2) {
+                    // both taps of a row from one pair of neighbouring
+                    // channels (sb, sb + 1), which holds sl and sr
+                    const int sb = min(sl, P.ns - 2);
+                    const bool l_hi = sl != sb, r_hi = sr != sb;
+                    const T* pl = src + (static_cast<size_t>(vl) * P.nt + it0) * P.ns + sb;
+                    const T* pr = src + (static_cast<size_t>(vr) * P.nt + it0) * P.ns + sb;
+                    for (int it = it0; it < it1; ++it, pl += P.ns, pr += P.ns) {
+                        T a[2], c[2];
+                        __builtin_memcpy(a, pl, sizeof(a));
+                        __builtin_memcpy(c, pr, sizeof(c));
+                        left_value += (1 - v) * static_cast<double>(l_hi ? a[1] : a[0]) +
+                                      v * static_cast<double>(r_hi ? a[1] : a[0]);
+                        right_value += (1 - v) * static_cast<double>(l_hi ? c[1] : c[0]) +
+                                       v * static_cast<double>(r_hi ? c[1] : c[0]);
+                        ++count;
+                    }
+                } else {
+                    const T* pl = src + (static_cast<size_t>(vl) * P.nt + it0) * P.ns;
+                    const T* pr = src + (static_cast<size_t>(vr) * P.nt + it0) * P.ns;
+                    for (int it = it0; it < it1; ++it, pl += P.ns, pr += P.ns) {
+                        left_value += (1 - v) * static_cast<double>(pl[sl]) +
+                                      v * static_cast<double>(pl[sr]);
+                        right_value += (1 - v) * static_cast<double>(pr[sl]) +
+                                       v * static_cast<double>(pr[sr]);
+                        ++count;
+                    }
                 }
                 cor_value = ((1 - u) * left_value + u * right_value) / count;
             }
