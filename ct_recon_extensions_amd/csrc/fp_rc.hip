@@ -6,7 +6,8 @@
 //
 // A wave holds 8 adjacent rays x 8 step phases: lane j of a ray samples the
 // ray's steps j, j + 8, j + 16, ..., so one gather instruction covers a
-// patch of 8 rays x 8 steps (8 x 1 pixels).  With one ray per lane, a
+// patch of 8 rays x 8 steps (8 x 1 pixels), and a sample's two taps in an
+// image row are one 8-byte load.  With one ray per lane, a
 // wave's 64 samples of one step lie on a segment across the rays that at
 // steep angles crosses one image row per lane (53 cache lines per gather
 // measured, which saturated the vector L1); the patch needs a few.
@@ -64,10 +65,25 @@ __global__ __launch_bounds__(256) void fp_rc_kernel(const T* __restrict__ img,
                 const int ix1 = static_cast<int>(fx), ix2 = static_cast<int>(ceil(x));
                 const int iy1 = static_cast<int>(fy), iy2 = static_cast<int>(ceil(y));
                 const double u = x - fx, v = y - fy;
-                const double p11 = static_cast<double>(im[ix1 + iy1 * P.nx]);
-                const double p21 = static_cast<double>(im[ix2 + iy1 * P.nx]);
-                const double p12 = static_cast<double>(im[ix1 + iy2 * P.nx]);
-                const double p22 = static_cast<double>(im[ix2 + iy2 * P.nx]);
+                double p11, p21, p12, p22;
+                if (P.nx >= 2) {
+                    // each row's two taps from one load of the column pair
+                    // (xb, xb + 1), which holds ix1 and ix2
+                    const int xb = min(ix1, P.nx - 2);
+                    const bool h1 = ix1 != xb, h2 = ix2 != xb;
+                    T q1[2], q2[2];
+                    __builtin_memcpy(q1, im + xb + iy1 * P.nx, sizeof(q1));
+                    __builtin_memcpy(q2, im + xb + iy2 * P.nx, sizeof(q2));
+                    p11 = static_cast<double>(h1 ? q1[1] : q1[0]);
+                    p21 = static_cast<double>(h2 ? q1[1] : q1[0]);
+                    p12 = static_cast<double>(h1 ? q2[1] : q2[0]);
+                    p22 = static_cast<double>(h2 ? q2[1] : q2[0]);
+                } else {
+                    p11 = static_cast<double>(im[ix1 + iy1 * P.nx]);
+                    p21 = static_cast<double>(im[ix2 + iy1 * P.nx]);
+                    p12 = static_cast<double>(im[ix1 + iy2 * P.nx]);
+                    p22 = static_cast<double>(im[ix2 + iy2 * P.nx]);
+                }
                 val = (1 - v) * ((1 - u) * p11 + u * p21) + v * ((1 - u) * p12 + u * p22);
             }
 #pragma unroll
