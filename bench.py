@@ -157,6 +157,13 @@ def profile_kernels(torch, F, which):
         x = torch.randn((1, c["n"], c["n"]), generator=g, device="cuda")
         for _ in range(3):
             img = F.parallel_projection_2d(x, **kw_rc())
+    elif which == "dd":
+        c = SVBP_CFG
+        x = torch.randn((8, c["n"], c["n"]), generator=g, device="cuda")
+        y = torch.randn((8, c["na"], c["ns"]), generator=g, device="cuda")
+        for _ in range(3):
+            img = F.parallel_projection_2d(x, **kw_dd())
+            img = F.parallel_projection_2d_grad(y, **kw_dd())
     elif which == "svbp":
         c = SVBP_CFG
         proj = torch.randn((1, c["na"], c["ns"]), generator=g, device="cuda")
@@ -259,6 +266,66 @@ def bench_raycasting(torch, F, reps, with_cpu=True):
         res.update(cpu_kind="reference" if kind == "ref" else "port",
                    cpu_1thread_ms=round((time.perf_counter() - t0) * 1e3, 1),
                    bit_identical=bool(np.array_equal(out[0].cpu().numpy(), ref)))
+    return res
+
+
+DD_SEED = 20261210
+
+
+def kw_dd():
+    return dict(kw_svbp(), method="disdriven")
+
+
+def bench_fp_dd(torch, F, reps, with_cpu=True):
+    """Distance-driven parallel FP and the gradient the reference defines for
+    it (src/fp_par_2d.cc:396-655) on the least-squares workload's geometry
+    (8 x 512^2, 720 views over 180 deg, 729 channels, ds = dx = 1): GPU time
+    per op call for the 8-slice batch, against the HBM roofline of its
+    algorithmic bytes (image and sinogram once each, plus the tables), and
+    the reference CPU path on one thread for slice 0 (bit-identity checked
+    there)."""
+    c = SVBP_CFG
+    n, na, ns, nb = c["n"], c["na"], c["ns"], 8
+    g = torch.Generator(device="cuda").manual_seed(DD_SEED)
+    img = torch.randn((nb, n, n), generator=g, device="cuda")
+    proj = torch.randn((nb, na, ns), generator=g, device="cuda")
+    out = F.parallel_projection_2d(img, **kw_dd())
+    gout = F.parallel_projection_2d_grad(proj, **kw_dd())
+    torch.cuda.synchronize()
+    ms_fp = time_events(lambda: F.parallel_projection_2d(img, **kw_dd()), reps, torch)
+    ms_gr = time_events(lambda: F.parallel_projection_2d_grad(proj, **kw_dd()), reps, torch)
+    alg_fp = nb * (n * n + na * ns) * 4 + na * ns * 3 * 8
+    alg_gr = nb * (n * n + na * ns) * 4 + na * n * 2 * 8
+    res = {"workload": "8 x 512^2, parallel 720 views/180deg, 729 ch, ds=dx=1, distance-driven; "
+                       "seeded N(0,1) image and sinogram", "seed": DD_SEED,
+           "fp_ms": round(ms_fp, 4), "grad_ms": round(ms_gr, 4),
+           "roofline_fp": {"kernel": "fp_dd_kernel", "bound": "hbm",
+                           "achieved": round(alg_fp / (ms_fp * 1e-3) / 1e9, 1),
+                           "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": round(alg_fp / (ms_fp * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                           "algorithmic_bytes": alg_fp},
+           "roofline_grad": {"kernel": "fp_dd_grad_kernel", "bound": "hbm",
+                             "achieved": round(alg_gr / (ms_gr * 1e-3) / 1e9, 1),
+                             "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": round(alg_gr / (ms_gr * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                             "algorithmic_bytes": alg_gr}}
+    if with_cpu:
+        from oracle import oracle as O
+        kind = "ref" if O.have_ref() else "oracle"
+        geo = O.geom(ns=ns, na=na, ds=1.0, orbit=math.pi / na, nx=n, ny=n, dx=1.0, dy=1.0)
+        t = O.fp_dd_prep(geo, kind=kind)
+        tg = O.fp_dd_grad_prep(geo, kind=kind)
+        x0, y0 = img[0].cpu().numpy(), proj[0].cpu().numpy()
+        t0 = time.perf_counter()
+        ref = O.fp_dd(geo, x0, t, kind=kind)
+        t1 = time.perf_counter()
+        gref = O.fp_dd_grad(geo, y0, tg, kind=kind)
+        t2 = time.perf_counter()
+        res.update(cpu_kind="reference" if kind == "ref" else "port",
+                   cpu_1thread_fp_ms_per_slice=round((t1 - t0) * 1e3, 1),
+                   cpu_1thread_grad_ms_per_slice=round((t2 - t1) * 1e3, 1),
+                   bit_identical_slice0=bool(np.array_equal(out[0].cpu().numpy(), ref) and
+                                             np.array_equal(gout[0].cpu().numpy(), gref)))
     return res
 
 
@@ -777,13 +844,15 @@ def main():
     ap.add_argument("--no-svbp", action="store_true", help="skip the single-view parallel BP")
     ap.add_argument("--no-rc", action="store_true", help="skip the ray-casting FP")
     ap.add_argument("--no-helical", action="store_true", help="skip the helical rebinning")
+    ap.add_argument("--no-dd", action="store_true",
+                    help="skip the distance-driven FP and its gradient")
     ap.add_argument("--no-bwd", action="store_true",
                     help="skip the differentiable FBP's backward (BP transpose)")
     ap.add_argument("--profile-bp", action="store_true",
                     help="counter-collection mode: three fan BP launches of the workload, then a "
                          "256 MiB device copy from a cold source (calibrates FETCH/WRITE_SIZE)")
     ap.add_argument("--profile", choices=("bp", "filter", "lsq", "svbp", "rc", "bpt", "ragged",
-                                          "filter5", "helical"), default=None,
+                                          "filter5", "helical", "dd"), default=None,
                     help="counter-collection mode for one kernel family: 'bp' as --profile-bp; "
                          "'filter' three fused fan-weighted ramp filters (default implementation) of the "
                          "workload; 'lsq' three FP and FP^T launches of the least-squares "
@@ -793,7 +862,8 @@ def main():
                          "deterministic); 'ragged' three eager runs of the ragged batch "
                          "(16x16 tiles, 32 views per round); 'filter5' three BF16x3 filters "
                          "of the 32 x 1440 x 1472 precision workload; 'helical' three 360LI "
-                         "and three 180LI rebinnings of the helical workload; each followed "
+                         "and three 180LI rebinnings of the helical workload; 'dd' three "
+                         "distance-driven FPs and gradients of their workload; each followed "
                          "by the calibration copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -991,6 +1061,11 @@ def main():
         rc_ = result["raycasting_fp"]
         print(f"[bench] ray-casting FP {rc_['gpu_ms']:.3f} ms, {rc_['gsamples_per_s']:.0f} "
               f"Gsamples/s", file=sys.stderr, flush=True)
+    if not args.no_dd:
+        result["distance_driven_fp"] = bench_fp_dd(torch, F, 10, with_cpu=not args.no_cpu)
+        dd = result["distance_driven_fp"]
+        print(f"[bench] distance-driven FP {dd['fp_ms']:.3f} ms, gradient {dd['grad_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
     if not args.no_helical:
         result["helical_rebinning"] = bench_helical(torch, F, 20, with_cpu=not args.no_cpu)
         hb = result["helical_rebinning"]

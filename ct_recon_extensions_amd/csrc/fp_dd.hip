@@ -113,8 +113,23 @@ __global__ __launch_bounds__(64) void fp_dd_kernel(const T* __restrict__ img,
             }
         }
         if (lsum > 0) {
+            if constexpr (std::is_same<T, float>::value) {
+                // The float quotients tsum / lsum, correctly rounded, from one
+                // FP64 reciprocal: q = tsum * RN(1 / lsum) in FP64 is within
+                // 2^-52 (relative) of the exact quotient, and a quotient of two
+                // floats is never an exact float midpoint and lies at least
+                // 2^-49 (relative) from every one (the midpoint has 25
+                // significant bits and lsum 24, so tsum - m lsum is a nonzero
+                // multiple of 2^-49 |m lsum|): rounding q to float gives the
+                // float division's result.
+                const double r = 1.0 / static_cast<double>(lsum);
 #pragma unroll
-            for (int k = 0; k < NB; ++k) sum[k] = sum[k] + tsum[k] / lsum;
+                for (int k = 0; k < NB; ++k)
+                    sum[k] = sum[k] + static_cast<float>(static_cast<double>(tsum[k]) * r);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NB; ++k) sum[k] = sum[k] + tsum[k] / lsum;
+            }
         }
         left += off;
         right += off;
