@@ -57,10 +57,13 @@ __global__ void dd_transpose_kernel(const T* __restrict__ src, T* __restrict__ d
     }
 }
 
-// One thread per (channel, view) and NB slices; data[b][u][v] is the image
+// One thread per (channel, view) and NB slices, a wavefront per 64 channels
+// of a view and 4 views per workgroup; data[b][u][v] is the image
 // (x-branch: u = iy, v = ix) or its transpose (y-branch: u = ix, v = iy).
+constexpr int kDdViews = 4;
+
 template <typename T, int NB>
-__global__ __launch_bounds__(64) void fp_dd_kernel(const T* __restrict__ img,
+__global__ __launch_bounds__(64 * kDdViews) void fp_dd_kernel(const T* __restrict__ img,
                                                    const T* __restrict__ imgT,
                                                    T* __restrict__ proj,
                                                    const double* __restrict__ sc,
@@ -68,8 +71,8 @@ __global__ __launch_bounds__(64) void fp_dd_kernel(const T* __restrict__ img,
                                                    const int* __restrict__ usex, DdProblem P)
 {
     const int is = blockIdx.x * 64 + threadIdx.x;
-    const int ia = blockIdx.y, b0 = blockIdx.z * NB;
-    if (is >= P.ns) return;
+    const int ia = blockIdx.y * kDdViews + threadIdx.y, b0 = blockIdx.z * NB;
+    if (is >= P.ns || ia >= P.na) return;
     const int nb = min(NB, P.nbatch - b0);
     const bool bx = usex[ia] != 0;
     const double dw = bx ? fabs(P.dy / sc[2 * ia + 1]) : fabs(P.dx / sc[2 * ia]);
@@ -207,9 +210,9 @@ template <typename T, int NB>
 int launch_fp_dd(const DdProblem& P, const T* img, const T* imgT, T* proj, const double* sc,
                  const double* bo, const int* usex, hipStream_t st)
 {
-    dim3 grid(div_up(P.ns, 64), P.na, div_up(P.nbatch, NB));
-    hipLaunchKernelGGL((fp_dd_kernel<T, NB>), grid, dim3(64), 0, st, img, imgT, proj, sc, bo,
-                       usex, P);
+    dim3 grid(div_up(P.ns, 64), div_up(P.na, kDdViews), div_up(P.nbatch, NB));
+    hipLaunchKernelGGL((fp_dd_kernel<T, NB>), grid, dim3(64, kDdViews), 0, st, img, imgT, proj,
+                       sc, bo, usex, P);
     return check_launch("fp_dd_kernel");
 }
 
