@@ -303,6 +303,9 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
             // ---- stage the image: rows u0 .. u0+rmax-1, pixels cw .. cw+WMAX-1.
             // A wave stages whole rows: one buffer descriptor per (row, slice
             // group), then K DMAs that differ only in the pixel block.
+            // Only the 16-pixel blocks up to the band's last candidate W1 are
+            // staged (no candidate reads past it).
+            const int kcnt = min(K, ((min(W1, cw + WMAX - 1) - cw) >> 4) + 1);
             if constexpr (S::kQuad) {
                 const int lane_px = lane >> 2;
                 const unsigned int lane_off = (lane & 3) * slice_bytes;
@@ -316,8 +319,7 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
                         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                             const_cast<float*>(base), 0, range, 0x00020000);
                         float* dst = simg + (r * S::kRowF4 + g * WMAX) * 4;
-#pragma unroll
-                        for (int k = 0; k < K; ++k) {
+                        for (int k = 0; k < kcnt; ++k) {
                             const int px = cw + 16 * k + lane_px;
                             const unsigned int voff =
                                 static_cast<unsigned int>(px) < static_cast<unsigned int>(nw)
