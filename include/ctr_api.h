@@ -135,11 +135,10 @@ int ctr_fan_bp_ex(const ctr_geom* g, int dtype, const void* proj, void* img, int
 /* Exact transpose of ctr_fan_bp: img [nbatch][ny][nx] -> proj
  * [nbatch][na][ns] (replaces FanBackprojection2DPixDrivenGrad<T>,
  * src/bp_fan_2d.cc:124-170, and the broken bp_fan_2d_angle transpose).
- * float32: view-parallel scatter; deterministic != 0 accumulates in 64-bit
- * fixed point with one scale per slice, so the result does not depend on
- * the order of the contributions (a slice holding Inf or NaN then yields
- * NaN throughout).  float64: a gather over the pixels that reach each
- * channel, in a fixed order, so it is bitwise repeatable in either mode. */
+ * Both types gather, per (view, channel), the pixels that reach it and sum
+ * them in registers in a fixed order (no atomics), so the result is
+ * bitwise repeatable in either mode; `deterministic` is kept for the
+ * reference's signature and changes nothing. */
 int ctr_fan_bp_t(const ctr_geom* g, int dtype, const void* img, void* proj, int nbatch,
                  const double* xpos, const double* ypos, const double* sincostbl, double scale,
                  int deterministic, void* stream);
