@@ -294,3 +294,60 @@ def test_fan_bpt_full_size_vs_oracle(F):
         assert err < TOL32, (b, err)
     single = host(F.fan_backprojection_2d_grad(cuda(img[None]), **fan_kwargs(geo)))
     assert rel_l2(single[0], ref) < TOL32
+
+
+# ------------------------------------ gather transposes: edge geometries
+# Channel counts from the smallest admitted (2) to around the 63-channel
+# workgroup (lane 0 owns the interval left of the workgroup's channels), a
+# single view and a few views, thin and one-pixel images, a wide fan whose
+# channels' level lines turn far across one workgroup, and detector offsets
+# that put the image's shadow partly off the detector.
+_EDGE_FAN = [
+    dict(ns=3, na=5, nx=9, ny=7, ds=3.0, dso=60.0, dsd=100.0, offset_s=0.0),
+    dict(ns=2, na=1, nx=13, ny=13, ds=2.0, dso=60.0, dsd=100.0, offset_s=0.3),
+    dict(ns=62, na=3, nx=1, ny=40, ds=0.9, dso=80.0, dsd=140.0, offset_s=-0.4),
+    dict(ns=63, na=7, nx=33, ny=1, ds=0.8, dso=80.0, dsd=140.0, offset_s=0.0),
+    dict(ns=64, na=16, nx=40, ny=24, ds=1.0, dso=70.0, dsd=120.0, offset_s=5.5),
+    dict(ns=127, na=9, nx=48, ny=48, ds=1.6, dso=60.0, dsd=90.0, offset_s=0.0),
+    dict(ns=190, na=33, nx=31, ny=57, ds=0.6, dso=120.0, dsd=200.0, offset_s=-20.0),
+]
+
+
+@pytest.mark.parametrize("k", range(len(_EDGE_FAN)))
+def test_fan_bpt_edge_geometries_vs_oracle(F, k):
+    e = _EDGE_FAN[k]
+    geo = dict(e, orbit=2 * math.pi / e["na"] * 0.93, orbit_start=0.37, dx=1.1, dy=0.9,
+               offset_x=0.3, offset_y=-0.2)
+    g = ogeom(geo)
+    t = O.fan_prep(g)
+    for nbatch in (1, 6):
+        img = randn(700 + 10 * k + nbatch, (nbatch, geo["ny"], geo["nx"]))
+        out = host(F.fan_backprojection_2d_grad(cuda(img), **fan_kwargs(geo)))
+        for b in range(nbatch):
+            ref = O.fan_bpt(g, img[b], t)
+            err = rel_l2(out[b], ref) if np.any(ref) else float(np.abs(out[b]).max())
+            assert err < TOL32, (k, nbatch, b, err)
+
+
+_EDGE_PAR = [
+    dict(ns=2, na=4, nx=6, ny=6, ds=4.0, offset_s=0.0),
+    dict(ns=63, na=1, nx=20, ny=11, ds=0.5, offset_s=0.25),
+    dict(ns=64, na=6, nx=1, ny=30, ds=0.7, offset_s=-3.0),
+    dict(ns=126, na=13, nx=45, ny=45, ds=0.6, offset_s=0.0),
+]
+
+
+@pytest.mark.parametrize("k", range(len(_EDGE_PAR)))
+def test_par_bpt_edge_geometries_vs_oracle(F, k):
+    e = _EDGE_PAR[k]
+    geo = dict(e, orbit=math.pi / e["na"] * 1.07, orbit_start=0.21, dx=1.0, dy=1.2,
+               offset_x=-0.4, offset_y=0.1)
+    g = ogeom(geo)
+    t = O.par_prep(g)
+    for nbatch in (1, 5):
+        img = randn(800 + 10 * k + nbatch, (nbatch, geo["ny"], geo["nx"]))
+        out = host(F.parallel_backprojection_2d_grad(cuda(img), **par_kwargs(geo)))
+        for b in range(nbatch):
+            ref = O.par_bpt(g, img[b], t)
+            err = rel_l2(out[b], ref) if np.any(ref) else float(np.abs(out[b]).max())
+            assert err < TOL32, (k, nbatch, b, err)
