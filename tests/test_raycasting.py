@@ -83,3 +83,21 @@ def test_raycasting_short_and_ragged_rays_bit_exact(F, dtype):
     out = F.parallel_projection_2d(torch.from_numpy(img).cuda(), **kwargs(GEO_SHORT)).cpu().numpy()
     for b in range(2):
         np.testing.assert_array_equal(out[b], O.fp_rc(g, img[b], t), err_msg=f"slice {b}")
+
+
+# Images one to three pixels wide or tall: a sample's two taps in a row come
+# from one load of the column pair (xb, xb + 1), xb = min(ix1, nx - 2), and a
+# one-pixel-wide image takes the scalar reads.
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny", [(1, 4), (2, 1), (2, 3), (3, 2)])
+def test_raycasting_tiny_images_bit_exact(F, nx, ny):
+    geo = dict(ns=9, na=12, ds=0.7, orbit=math.pi / 12, offset_s=0.1, orbit_start=0.05, nx=nx,
+               ny=ny, dx=1.0, dy=1.0, offset_x=0.0, offset_y=0.0, fov=3.0)
+    g = O.geom(**geo)
+    t = O.fp_rc_prep(g)
+    for dtype in (np.float32, np.float64):
+        img = randn(93 + nx * 7 + ny, (2, ny, nx), dtype)
+        out = F.parallel_projection_2d(torch.from_numpy(img).cuda(), **kwargs(geo)).cpu().numpy()
+        for b in range(2):
+            np.testing.assert_array_equal(out[b], O.fp_rc(g, img[b], t),
+                                          err_msg=f"{nx}x{ny} {dtype.__name__} slice {b}")
