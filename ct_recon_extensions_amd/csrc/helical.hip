@@ -67,13 +67,19 @@ __device__ __forceinline__ double heli_mean(const HeliProblem& P, const T* __res
                                             int view, int is, int it0, int it1)
 {
     const T* p = in + (static_cast<size_t>(view) * P.nt + it0) * P.ns + is;
+    const int n = it1 - it0;
     double v = 0;
-    int count = 0;
-    for (int it = it0; it < it1; ++it, p += P.ns) {
-        v += static_cast<double>(*p);
-        ++count;
+    // four rows' loads issued together (rows past the window re-read its
+    // last row and are not added), then added in row order
+    for (int k = 0; k < n; k += 4) {
+        T a[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] = p[static_cast<size_t>(min(k + j, n - 1)) * P.ns];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k + j < n) v += static_cast<double>(a[j]);
     }
-    return v / count;
+    return v / n;
 }
 
 // One wavefront per output row (slice, view): the row's view numbers, z
