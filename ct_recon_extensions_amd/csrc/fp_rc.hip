@@ -16,8 +16,8 @@
 // same order, rounded to T once, so the result matches it bit for bit:
 // - every lane advances its ray one step at a time, as the reference
 //   accumulates x += step sin;
-// - the 8 lanes of a ray sum the ray's samples in step order with width-8
-//   shuffles, each lane computing the same sum;
+// - the 8 lanes of a ray sum the ray's samples in step order, read back
+//   through LDS, each lane computing the same sum;
 // - a step past the ray's end or outside the image contributes +0.  The sum
 //   starts at +0 and never becomes -0 under round-to-nearest, so adding +0
 //   leaves it unchanged.
@@ -43,6 +43,8 @@ __global__ __launch_bounds__(256) void fp_rc_kernel(const T* __restrict__ img,
                                                     const RcProblem P)
 {
 #pragma clang fp contract(off)
+    // each lane's sample value, read back by the 8 lanes of its ray
+    __shared__ __align__(16) double vals[256];
     const int j = threadIdx.x % kRcPhases;
     const int r = blockIdx.x * kRcRays + threadIdx.x / kRcPhases;
     const bool valid = r < P.nrays;
@@ -86,8 +88,20 @@ __global__ __launch_bounds__(256) void fp_rc_kernel(const T* __restrict__ img,
                 }
                 val = (1 - v) * ((1 - u) * p11 + u * p21) + v * ((1 - u) * p12 + u * p22);
             }
+            // the ray's 8 values through LDS (one write, four 16-byte reads;
+            // a wave's LDS accesses stay in order), added in step order
+            vals[threadIdx.x] = val;
+            __builtin_amdgcn_wave_barrier();
+            const double2* g2 = reinterpret_cast<const double2*>(vals + (threadIdx.x & ~(kRcPhases - 1)));
+            double2 w[kRcPhases / 2];
 #pragma unroll
-            for (int t = 0; t < kRcPhases; ++t) sum += __shfl(val, t, kRcPhases);
+            for (int t = 0; t < kRcPhases / 2; ++t) w[t] = g2[t];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int t = 0; t < kRcPhases / 2; ++t) {
+                sum += w[t].x;
+                sum += w[t].y;
+            }
 #pragma unroll
             for (int t = 0; t < kRcPhases; ++t) {
                 x += stx;
