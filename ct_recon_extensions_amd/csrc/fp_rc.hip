@@ -60,39 +60,13 @@ __global__ __launch_bounds__(256) void fp_rc_kernel(const T* __restrict__ img,
             y += sty;
         }
         double sum = 0;
-        for (int k0 = 0; k0 < n; k0 += kRcPhases) {
-            double val = 0;
-            if (k0 + j < n && x >= 0 && x <= P.xmax && y >= 0 && y <= P.ymax) {
-                const double fx = floor(x), fy = floor(y);
-                const int ix1 = static_cast<int>(fx), ix2 = static_cast<int>(ceil(x));
-                const int iy1 = static_cast<int>(fy), iy2 = static_cast<int>(ceil(y));
-                const double u = x - fx, v = y - fy;
-                double p11, p21, p12, p22;
-                if (P.nx >= 2) {
-                    // each row's two taps from one load of the column pair
-                    // (xb, xb + 1), which holds ix1 and ix2
-                    const int xb = min(ix1, P.nx - 2);
-                    const bool h1 = ix1 != xb, h2 = ix2 != xb;
-                    T q1[2], q2[2];
-                    __builtin_memcpy(q1, im + xb + iy1 * P.nx, sizeof(q1));
-                    __builtin_memcpy(q2, im + xb + iy2 * P.nx, sizeof(q2));
-                    p11 = static_cast<double>(h1 ? q1[1] : q1[0]);
-                    p21 = static_cast<double>(h2 ? q1[1] : q1[0]);
-                    p12 = static_cast<double>(h1 ? q2[1] : q2[0]);
-                    p22 = static_cast<double>(h2 ? q2[1] : q2[0]);
-                } else {
-                    p11 = static_cast<double>(im[ix1 + iy1 * P.nx]);
-                    p21 = static_cast<double>(im[ix2 + iy1 * P.nx]);
-                    p12 = static_cast<double>(im[ix1 + iy2 * P.nx]);
-                    p22 = static_cast<double>(im[ix2 + iy2 * P.nx]);
-                }
-                val = (1 - v) * ((1 - u) * p11 + u * p21) + v * ((1 - u) * p12 + u * p22);
-            }
-            // the ray's 8 values through LDS (one write, four 16-byte reads;
-            // a wave's LDS accesses stay in order), added in step order
+        // the ray's 8 values through LDS (one write, four 16-byte reads; a
+        // wave's LDS accesses stay in order), added in step order
+        auto add_ray = [&](double val) {
             vals[threadIdx.x] = val;
             __builtin_amdgcn_wave_barrier();
-            const double2* g2 = reinterpret_cast<const double2*>(vals + (threadIdx.x & ~(kRcPhases - 1)));
+            const double2* g2 =
+                reinterpret_cast<const double2*>(vals + (threadIdx.x & ~(kRcPhases - 1)));
             double2 w[kRcPhases / 2];
 #pragma unroll
             for (int t = 0; t < kRcPhases / 2; ++t) w[t] = g2[t];
@@ -102,10 +76,79 @@ __global__ __launch_bounds__(256) void fp_rc_kernel(const T* __restrict__ img,
                 sum += w[t].x;
                 sum += w[t].y;
             }
+        };
+        auto advance = [&]() {
 #pragma unroll
             for (int t = 0; t < kRcPhases; ++t) {
                 x += stx;
                 y += sty;
+            }
+        };
+        if (P.nx >= 2) {
+            // Software-pipelined over two register sets: the next sample's
+            // taps are loaded before this sample's value and the ray's sum.
+            // A sample's two taps in a row come from one load of the column
+            // pair (xb, xb + 1), which holds ix1 and ix2.
+            struct Tap {
+                bool in, h1, h2;
+                double u, v;
+                T q1[2], q2[2];
+            };
+            auto fetch = [&](int k, Tap& s) {
+                s.in = k + j < n && x >= 0 && x <= P.xmax && y >= 0 && y <= P.ymax;
+                if (s.in) {
+                    const double fx = floor(x), fy = floor(y);
+                    const int ix1 = static_cast<int>(fx), ix2 = static_cast<int>(ceil(x));
+                    const int iy1 = static_cast<int>(fy), iy2 = static_cast<int>(ceil(y));
+                    s.u = x - fx;
+                    s.v = y - fy;
+                    const int xb = min(ix1, P.nx - 2);
+                    s.h1 = ix1 != xb;
+                    s.h2 = ix2 != xb;
+                    __builtin_memcpy(s.q1, im + xb + iy1 * P.nx, sizeof(s.q1));
+                    __builtin_memcpy(s.q2, im + xb + iy2 * P.nx, sizeof(s.q2));
+                }
+            };
+            auto value = [&](const Tap& s) {
+                double val = 0;
+                if (s.in) {
+                    const double u = s.u, v = s.v;
+                    const double p11 = static_cast<double>(s.h1 ? s.q1[1] : s.q1[0]);
+                    const double p21 = static_cast<double>(s.h2 ? s.q1[1] : s.q1[0]);
+                    const double p12 = static_cast<double>(s.h1 ? s.q2[1] : s.q2[0]);
+                    const double p22 = static_cast<double>(s.h2 ? s.q2[1] : s.q2[0]);
+                    val = (1 - v) * ((1 - u) * p11 + u * p21) + v * ((1 - u) * p12 + u * p22);
+                }
+                return val;
+            };
+            Tap ta, tb;
+            fetch(0, ta);
+            for (int k0 = 0;; k0 += 2 * kRcPhases) {
+                advance();
+                fetch(k0 + kRcPhases, tb);
+                add_ray(value(ta));
+                if (k0 + kRcPhases >= n) break;
+                advance();
+                fetch(k0 + 2 * kRcPhases, ta);
+                add_ray(value(tb));
+                if (k0 + 2 * kRcPhases >= n) break;
+            }
+        } else {
+            for (int k0 = 0; k0 < n; k0 += kRcPhases) {
+                double val = 0;
+                if (k0 + j < n && x >= 0 && x <= P.xmax && y >= 0 && y <= P.ymax) {
+                    const double fx = floor(x), fy = floor(y);
+                    const int ix1 = static_cast<int>(fx), ix2 = static_cast<int>(ceil(x));
+                    const int iy1 = static_cast<int>(fy), iy2 = static_cast<int>(ceil(y));
+                    const double u = x - fx, v = y - fy;
+                    const double p11 = static_cast<double>(im[ix1 + iy1 * P.nx]);
+                    const double p21 = static_cast<double>(im[ix2 + iy1 * P.nx]);
+                    const double p12 = static_cast<double>(im[ix1 + iy2 * P.nx]);
+                    const double p22 = static_cast<double>(im[ix2 + iy2 * P.nx]);
+                    val = (1 - v) * ((1 - u) * p11 + u * p21) + v * ((1 - u) * p12 + u * p22);
+                }
+                add_ray(val);
+                advance();
             }
         }
         if (j == 0) proj[static_cast<size_t>(b) * P.nrays + r] = static_cast<T>(sum * P.step * P.dx);
