@@ -97,6 +97,12 @@ struct GatherLaunch {
     double scale;
 };
 
+// x of lane `lane` (uniform) of the wave, through a scalar register.
+__device__ __forceinline__ float lane_value(float x, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+
 // One linear constraint of a lane's candidates along a row: w >= A + B u
 // (lower) or w <= A + B u (upper); none = no constraint from this line.
 struct GatherBound {
@@ -256,8 +262,8 @@ void bp_gather_t_kernel(const GatherLaunch<Geo> L)
         if (hull) {
             // one pixel of margin on each side covers the rounding of the
             // lanes' FP32 bounds about the hull
-            const float l0 = __shfl(lo, 0), l1 = __shfl(lo, last);
-            const float h0 = __shfl(hi, 0), h1 = __shfl(hi, last);
+            const float l0 = lane_value(lo, 0), l1 = lane_value(lo, last);
+            const float h0 = lane_value(hi, 0), h1 = lane_value(hi, last);
             ilo = static_cast<int>(floorf(fmaxf(fminf(l0, l1), -2.0f))) - 1;
             ihi = static_cast<int>(ceilf(fminf(fmaxf(h0, h1), static_cast<float>(nw + 1)))) + 1;
             ilo = max(ilo, 0);
