@@ -104,10 +104,19 @@ __global__ __launch_bounds__(64 * kDdViews) void fp_dd_kernel(const T* __restric
             for (unsigned int v = static_cast<unsigned int>(fl); static_cast<double>(v) < vend; ++v) {
                 const double len = fmin(static_cast<double>(v + 1), right) -
                                    fmax(static_cast<double>(v), left);
+                // every slice's cell pair loaded first (a slice past the
+                // batch re-reads the last one and is not added)
+                T c0[NB], c1[NB];
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const T* q = row + static_cast<size_t>(min(k, nb - 1)) * slice + v;
+                    c0[k] = q[0];
+                    c1[k] = q[1];
+                }
 #pragma unroll
                 for (int k = 0; k < NB; ++k) {
                     if (k < nb) {
-                        const T pair = row[k * slice + v] + row[k * slice + v + 1];
+                        const T pair = c0[k] + c1[k];
                         tsum[k] = static_cast<T>(static_cast<double>(tsum[k]) +
                                                  len * static_cast<double>(pair) / 2);
                     }
@@ -192,9 +201,15 @@ __global__ __launch_bounds__(256) void fp_dd_grad_kernel(const T* __restrict__ p
                  ++is) {
                 const double len = fmin(static_cast<double>(is) + 0.5, right) -
                                    fmax(static_cast<double>(is) - 0.5, left);
+                // every slice's value loaded first (a slice past the batch
+                // re-reads the last one and is not added)
+                T val[NB];
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
-                    if (k < nb) tmp[k] += len * static_cast<double>(row[k * sslice + is]) / 2;
+                    val[k] = row[static_cast<size_t>(min(k, nb - 1)) * sslice + is];
+#pragma unroll
+                for (int k = 0; k < NB; ++k)
+                    if (k < nb) tmp[k] += len * static_cast<double>(val[k]) / 2;
             }
         }
         const double w = weights[ia];
