@@ -96,11 +96,19 @@ __global__ __launch_bounds__(256) void fp_kernel(const T* __restrict__ img,
             const int i2 = min(i1 + 1, rowlen - 1);
             const Acc u = static_cast<Acc>(x - fl);
             const T* row = src + static_cast<size_t>(k) * rowlen;
+            // every slice's taps loaded first (a slice past the batch
+            // re-reads the last one and is not added)
+            T t0[NB], t1[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                t0[b] = row[(b0 + min(b, nb - 1)) * slice + i1];
+                t1[b] = row[(b0 + min(b, nb - 1)) * slice + i2];
+            }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 if (b < nb) {
-                    const Acc p0 = row[(b0 + b) * slice + i1];
-                    const Acc p1 = row[(b0 + b) * slice + i2];
+                    const Acc p0 = t0[b];
+                    const Acc p1 = t1[b];
                     sum[b] += p0 + u * (p1 - p0);
                 }
             }
@@ -321,9 +329,14 @@ __device__ __forceinline__ void fpt_view(const T* __restrict__ proj,
     for (int c = c0; c <= c1; ++c) {
         const Acc w = static_cast<Acc>(1.0 - fabs((mid - c) / o1));
         const T* pr = proj + static_cast<size_t>(a) * P.ns + c;
+        // every slice's value loaded first (a slice past the batch re-reads
+        // the last one and is not added)
+        T val[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) val[b] = pr[(b0 + min(b, nb - 1)) * pslice];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
-            if (b < nb) tmp[b] += w * static_cast<Acc>(pr[(b0 + b) * pslice]);
+            if (b < nb) tmp[b] += w * static_cast<Acc>(val[b]);
     }
     const double wa = weights[a];
 #pragma unroll

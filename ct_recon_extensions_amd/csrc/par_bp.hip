@@ -167,9 +167,15 @@ __global__ void par_bpt_f64_gather_kernel(const double* __restrict__ img,
             const double wc = (is1 == c ? 1 - u : 0.0) + (is2 == c ? u : 0.0);
             if (wc == 0.0) continue;
             const size_t pix = static_cast<size_t>(iy) * P.nx + ix;
+            // every slice's pixel loaded first (a slice past the batch
+            // re-reads the last one and is not added)
+            double pv[kParGatherSlices];
 #pragma unroll
             for (int b = 0; b < kParGatherSlices; ++b)
-                if (b < nb) acc[b] += wc * img[static_cast<size_t>(b0 + b) * P.nx * P.ny + pix];
+                pv[b] = img[static_cast<size_t>(b0 + min(b, nb - 1)) * P.nx * P.ny + pix];
+#pragma unroll
+            for (int b = 0; b < kParGatherSlices; ++b)
+                if (b < nb) acc[b] += wc * pv[b];
         }
     }
 #pragma unroll
