@@ -369,15 +369,19 @@ __global__ __launch_bounds__(256) void fpt_kernel(const T* __restrict__ proj,
 
 // The border pixels (nx, ny >= 3) after the engine pass, in two steps so
 // that neighbouring threads read neighbouring channels: (1) thread = border
-// pixel, a chunk of kBorderViews views per workgroup row, FP64 partial sums
-// per (view chunk, slice, pixel); (2) the partials added in chunk order.
+// pixel, a chunk of views per workgroup row, FP64 partial sums per (view
+// chunk, slice, pixel); (2) the partials added in chunk order.  Chunks of
+// kBorderViews views where the partials live in a caller's workspace (the
+// allocation-free entry point's fixed scratch), of kBorderViewsAlloc views
+// (four times the workgroups) where they are allocated per call.
 constexpr int kBorderViews = 16;
+constexpr int kBorderViewsAlloc = 4;
 
 template <int NB>
 __global__ __launch_bounds__(256) void fpt_border_partial_kernel(
     const float* __restrict__ proj, double* __restrict__ partial,
     const double* __restrict__ weights, const double* __restrict__ pos,
-    const int* __restrict__ usex, FpProblem P, int stride, int nborder)
+    const int* __restrict__ usex, FpProblem P, int stride, int nborder, int cviews)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int vc = blockIdx.y;
@@ -389,8 +393,8 @@ __global__ __launch_bounds__(256) void fpt_border_partial_kernel(
     double sum[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) sum[b] = 0.0;
-    const int a1 = min(P.na, (vc + 1) * kBorderViews);
-    for (int a = vc * kBorderViews; a < a1; ++a)
+    const int a1 = min(P.na, (vc + 1) * cviews);
+    for (int a = vc * cviews; a < a1; ++a)
         fpt_view<float, NB>(proj, weights, pos, usex, P, stride, a, ix, iy, b0, nb, sum);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
@@ -532,11 +536,12 @@ int launch_fpt(const FpProblem& P, const T* proj, T* img, const double* w, const
     return check_launch("fpt_kernel");
 }
 
-// Bytes of FP64 partials the border pass needs (nchunks x nbatch x nborder).
-size_t fpt_border_bytes(int nx, int ny, int na, int nbatch)
+// Bytes of FP64 partials the border pass needs (nchunks x nbatch x nborder)
+// with chunks of cviews views.
+size_t fpt_border_bytes(int nx, int ny, int na, int nbatch, int cviews = kBorderViews)
 {
     const size_t nborder = 2 * static_cast<size_t>(nx) + 2 * (static_cast<size_t>(ny) - 2);
-    return static_cast<size_t>(div_up(na, kBorderViews)) * nbatch * nborder * sizeof(double);
+    return static_cast<size_t>(div_up(na, cviews)) * nbatch * nborder * sizeof(double);
 }
 
 // ws: caller workspace of at least fpt_border_bytes(...) bytes, or null for a
@@ -546,14 +551,15 @@ int launch_fpt_border(const FpProblem& P, const float* proj, float* img, const d
                       const double* pos, const int* usex, int stride, void* ws, hipStream_t st)
 {
     const int nborder = 2 * P.nx + 2 * (P.ny - 2);
-    const int nchunks = div_up(P.na, kBorderViews);
+    const int cviews = ws ? kBorderViews : kBorderViewsAlloc;
+    const int nchunks = div_up(P.na, cviews);
     double* partial = static_cast<double*>(ws);
     if (!ws)
         CTR_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial),
-                               fpt_border_bytes(P.nx, P.ny, P.na, P.nbatch), st));
+                               fpt_border_bytes(P.nx, P.ny, P.na, P.nbatch, cviews), st));
     hipLaunchKernelGGL((fpt_border_partial_kernel<NB>),
                        dim3(div_up(nborder, 256), nchunks, div_up(P.nbatch, NB)), dim3(256), 0, st,
-                       proj, partial, w, pos, usex, P, stride, nborder);
+                       proj, partial, w, pos, usex, P, stride, nborder, cviews);
     int rc = check_launch("fpt_border_partial_kernel");
     if (rc == CTR_OK) {
         hipLaunchKernelGGL(fpt_border_reduce_kernel, dim3(div_up(nborder, 256), P.nbatch),
