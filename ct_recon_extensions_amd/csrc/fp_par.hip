@@ -410,8 +410,19 @@ __global__ void fpt_border_reduce_kernel(const double* __restrict__ partial,
     if (i >= nborder) return;
     int ix, iy;
     border_pixel(i, P.nx, P.ny, ix, iy);
+    // eight chunks' partials loaded together (chunks past the last re-read
+    // it and are not added), then added in chunk order
+    const double* q = partial + static_cast<size_t>(b) * nborder + i;
+    const size_t cstride = static_cast<size_t>(P.nbatch) * nborder;
     double t = 0.0;
-    for (int c = 0; c < nchunks; ++c) t += partial[(static_cast<size_t>(c) * P.nbatch + b) * nborder + i];
+    for (int c = 0; c < nchunks; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = q[static_cast<size_t>(min(c + k, nchunks - 1)) * cstride];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (c + k < nchunks) t += v[k];
+    }
     img[(static_cast<size_t>(b) * P.ny + iy) * P.nx + ix] = static_cast<float>(t);
 }
 
