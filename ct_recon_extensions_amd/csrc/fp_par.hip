@@ -128,12 +128,15 @@ __global__ __launch_bounds__(256) void fp_kernel(const T* __restrict__ img,
 // transpose (y-branch views) in chunks of kFpRows: per chunk it stages, for
 // all NB slices, the span of columns its rays can touch (DMA from global
 // straight into LDS, zero outside the row; slice rows kFpSpan words apart),
-// then every thread advances its V rays through those rows.  A staged pixel
-// is read by ~V rays instead of every view re-reading the image through
-// L1/L2, and a ray's two taps for a pair of slices come from one
-// ds_read2st64_b32.  Each ray keeps the reference's FP64 position
-// accumulation (x += offset per row) and inclusion test; the lerp and the
-// sum are FP32.  Views of the other branch in the group take a second pass.
+// then every thread advances its V rays through those rows.  Two chunk
+// buffers alternate: the DMAs of chunk h + 1 are issued before the rays walk
+// chunk h, so their latency hides behind that walk and one barrier per chunk
+// separates the two.  A staged pixel is read by ~V rays instead of every view
+// re-reading the image through L1/L2, and a ray's two taps for a pair of
+// slices come from one ds_read2st64_b32.  Each ray keeps the reference's FP64
+// position accumulation (x += offset per row) and inclusion test; the lerp
+// and the sum are FP32.  Views of the other branch in the group take a second
+// pass.
 template <int NB, int V, int kFpRows, int kFpSpan>
 __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ img,
                                                      const float* __restrict__ imgT,
@@ -144,7 +147,8 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
                                                      int cpb)
 {
     static_assert(NB % 2 == 0 || NB == 1, "slices are read in pairs");
-    __shared__ __align__(16) float rows[kFpRows * NB * kFpSpan];
+    constexpr int kChunk = kFpRows * NB * kFpSpan;  // floats of one chunk buffer
+    __shared__ __align__(16) float rows[2 * kChunk];
     // the wave index is uniform: keeps the DMA's buffer descriptor in SGPRs
     // (no per-lane waterfall around buffer_load ... lds)
     const int tid = threadIdx.x, lane = tid & 63, wave = uniform_i(tid >> 6);
@@ -155,6 +159,11 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
     const int b0 = blockIdx.z * NB;
     const int nb = min(NB, P.nbatch - b0);
     const size_t slice = static_cast<size_t>(P.nx) * P.ny;
+    // right tap through an offset the compiler cannot fold, so the slice
+    // pairs load as ds_read2st64_b32 (rows kFpSpan apart); at x == hi its
+    // weight u is 0 and it reads a staged zero
+    int one = 1;
+    asm volatile("" : "+v"(one));
 
     for (int pass = 0; pass < 2; ++pass) {
         const bool xb = pass == 0;
@@ -171,7 +180,11 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
         const int rowlen = xb ? P.nx : P.ny;
         const float* src = xb ? img : imgT;
         const double hi = rowlen - 1;
-        double x[V], off[V], blo[V], bhi[V];
+        double x[V], off[V];
+        // view v's rays cover columns [clo[v] + k0 * coff[v], chi[v] + k0 * coff[v]]
+        // over rows k0 .. k0 + kFpRows - 1 (FP32 bounds: their error is far
+        // below the column of margin each side)
+        float clo[V], chi[V], coff[V];
         float sum[V][NB];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -179,37 +192,36 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
             const double* r = bo + 2 * static_cast<size_t>(a) * P.ns;
             x[v] = has_ray ? r[2 * s] : 0.0;
             off[v] = r[1];
-            blo[v] = r[2 * s0];
-            bhi[v] = r[2 * slast];
+            const double blo = r[2 * s0], bhi = r[2 * slast];
+            const double drift = (kFpRows - 1) * off[v];
+            clo[v] = act[v] ? static_cast<float>(fmin(blo, bhi) + fmin(0.0, drift)) : 1e30f;
+            chi[v] = act[v] ? static_cast<float>(fmax(blo, bhi) + fmax(0.0, drift)) : -1e30f;
+            coff[v] = static_cast<float>(off[v]);
 #pragma unroll
             for (int b = 0; b < NB; ++b) sum[v][b] = 0.0f;
         }
-        for (int k0 = 0; k0 < nsteps; k0 += kFpRows) {
-            const int nr = min(kFpRows, nsteps - k0);
-            // columns the chunk's rays can touch (one column of margin each side)
-            double xmin = 1e300, xmax = -1e300;
+        // columns cmin .. cmin + 64 * nload - 1 of the chunk at row k0, or
+        // false when no ray of the chunk is on the image
+        auto span = [&](int k0, int& cmin, int& nload) -> bool {
+            const float k = static_cast<float>(k0);
+            float xmin = 1e30f, xmax = -1e30f;
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                if (!act[v]) continue;
-                const double d0 = k0 * off[v], d1 = (k0 + nr - 1) * off[v];
-                xmin = fmin(xmin, fmin(blo[v], bhi[v]) + fmin(d0, d1));
-                xmax = fmax(xmax, fmax(blo[v], bhi[v]) + fmax(d0, d1));
+                xmin = fminf(xmin, fmaf(k, coff[v], clo[v]));
+                xmax = fmaxf(xmax, fmaf(k, coff[v], chi[v]));
             }
-            __syncthreads();  // previous chunk consumed
-            if (xmax < -1.0 || xmin > hi + 1.0) {
-                // no ray of the chunk is on the image: only advance them
-#pragma unroll
-                for (int v = 0; v < V; ++v)
-                    for (int k = 0; k < nr; ++k) x[v] += off[v];
-                continue;
-            }
-            const int cmin = max(0, static_cast<int>(floor(xmin)) - 1);
-            const int cmax = min(rowlen - 1, static_cast<int>(floor(xmax)) + 2);
-            // 64-column DMA rows covering columns cmin .. cmax + 1 (the right tap
-            // of a sample at x == nx-1 reads cmax + 1: zero past the row); the
-            // host keeps cmax - cmin + 2 <= kFpSpan
-            const int nload = min((cmax - cmin + 65) / 64, kFpSpan / 64);
-            // stage rows (k0 + kk, slice b) -> rows[(kk * NB + b) * kFpSpan + c]
+            if (xmax < -1.0f || xmin > static_cast<float>(hi) + 1.0f) return false;
+            // one column of margin each side; the right tap of a sample at
+            // x == nx-1 reads cmax + 1 (zero past the row); the host keeps
+            // cmax - cmin + 2 <= kFpSpan
+            cmin = max(0, static_cast<int>(floorf(xmin)) - 1);
+            const int cmax = min(rowlen - 1, static_cast<int>(floorf(xmax)) + 2);
+            nload = min((cmax - cmin + 65) / 64, kFpSpan / 64);
+            return true;
+        };
+        // rows (k0 + kk, slice b) -> buf[(kk * NB + b) * kFpSpan + c - cmin]
+        auto stage = [&](int k0, int cmin, int nload, float* buf) {
+            const int nr = min(kFpRows, nsteps - k0);
             for (int e = wave; e < nr * NB; e += 4) {
                 const int kk = e / NB, b = e - kk * NB;
                 const float* rowp = src + static_cast<size_t>(b0 + b) * slice +
@@ -217,23 +229,37 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
                 const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                     const_cast<float*>(rowp), 0,
                     b < nb ? static_cast<unsigned int>(rowlen) * 4u : 0u, 0x00020000);
-                float* dst = rows + e * kFpSpan;
+                float* dst = buf + e * kFpSpan;
                 for (int c = 0; c < nload; ++c)
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(
                         rs, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4,
                         (cmin + 64 * c + lane) * 4, 0, 0, 0);
             }
+        };
+        const int nchunks = (nsteps + kFpRows - 1) / kFpRows;
+        int cm = 0, nl = 0;
+        __syncthreads();  // the previous pass has finished reading both buffers
+        bool on = span(0, cm, nl);
+        if (on) stage(0, cm, nl, rows);
+        for (int h = 0; h < nchunks; ++h) {
+            const int k0 = h * kFpRows;
+            const int nr = min(kFpRows, nsteps - k0);
+            const float* cur = rows + (h & 1) * kChunk;
+            // this wave's DMAs of chunk h have landed (none of chunk h + 1 is
+            // issued yet); after the barrier every wave's have, and every
+            // wave is done with chunk h - 1, whose buffer takes chunk h + 1
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            if (has_ray) {
-                // right tap through an offset the compiler cannot fold, so the
-                // slice pairs load as ds_read2st64_b32 (rows kFpSpan apart); at
-                // x == hi its weight u is 0 and it reads a staged zero
-                int one = 1;
-                asm volatile("" : "+v"(one));
+            int cm_next = 0, nl_next = 0;
+            bool on_next = false;
+            if (h + 1 < nchunks) {
+                on_next = span(k0 + kFpRows, cm_next, nl_next);
+                if (on_next) stage(k0 + kFpRows, cm_next, nl_next, rows + ((h + 1) & 1) * kChunk);
+            }
+            if (on && has_ray) {
 #pragma unroll 1
                 for (int kk = 0; kk < nr; ++kk) {
-                    const float* rk = rows + kk * NB * kFpSpan - cmin;
+                    const float* rk = cur + kk * NB * kFpSpan - cm;
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
                         if (act[v] && x[v] >= 0 && x[v] <= hi) {
@@ -251,10 +277,14 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(const float* __restrict__ i
                     }
                 }
             } else {
+                // no ray of the chunk is on the image (or no ray here): only
+                // advance them
 #pragma unroll
                 for (int v = 0; v < V; ++v)
                     for (int k = 0; k < nr; ++k) x[v] += off[v];
             }
+            on = on_next;
+            cm = cm_next;
         }
         if (has_ray) {
 #pragma unroll
@@ -443,10 +473,15 @@ int launch_fp(const FpProblem& P, const T* img, const T* imgT, T* proj, const do
 // fewer (fp_views) so that the grid still fills the GPU.
 constexpr int kFpViews = 4;
 constexpr int kFpSlices = 4;
-// 4-row chunks with 512-column spans (32 KB of LDS at 4 slices, 5 workgroups
-// per CU) measured 0.88 ms at two views against 0.95 ms for 8-row chunks with
-// 384-column spans (48 KB, 3 workgroups per CU).
-constexpr int kFpRowsDefault = 4, kFpSpanDefault = 512;
+// Two buffers of 2-row chunks with 512-column spans (32 KB of LDS at 4
+// slices; 97 VGPRs, four waves per SIMD): 0.775 ms on that scan, against
+// 0.84 ms for one buffer of 4-row chunks (no overlap of the DMAs with the
+// walk), 0.99 ms for two buffers of 4 rows (64 KB, two workgroups per CU),
+// 0.79 ms for two of 1 row and 0.88 ms with the VGPRs held to 95 for five
+// waves per SIMD.  Taking the taps of all V views branch-free (rays off the
+// image reading a zero pad) so their LDS reads overlap measured 0.84 ms:
+// whole waves then no longer skip the views they miss.
+constexpr int kFpRowsDefault = 2, kFpSpanDefault = 512;
 
 // Columns the rays of one fp_lds_kernel chunk can span: cpb channels at up
 // to sqrt(2) ds / pixel per channel, kFpRows rows at up to one column per row
