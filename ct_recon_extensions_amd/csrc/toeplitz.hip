@@ -671,10 +671,9 @@ ramp_rows_bf16x3_kernel(const ToeplitzParams P)
 // (hi and lo planes), so a B fragment of 8 consecutive taps is two aligned
 // 8-byte reads per plane, and G's Toeplitz structure lets a wave load only
 // two new fragments per K step (a ring of TPW fragments).  The next block's
-// rows arrive in registers (U floats per lane, the block's contiguous
-// kRows*ns floats split linearly over the lanes) while the current block
-// computes, and the results leave through the row buffer as whole coalesced
-// rows.  NW/4 waves per SIMD (256 / (NW/4) VGPRs each).
+// rows arrive in registers (U floats per lane: each of the kRows rows as
+// U / kRows chunks of NT channels) while the current block computes, and the
+// results leave through the row buffer as whole coalesced rows.  NW/4 waves per SIMD (256 / (NW/4) VGPRs each).
 constexpr int kChunk = 64;              // inputs per A chunk (two K steps)
 constexpr int kAC = kChunk + 8;         // bf16 row stride of an A plane (144 B)
 constexpr int kAPlane = kRows * kAC;    // bf16 per plane
@@ -741,28 +740,25 @@ ramp_persist_bf16x3_kernel(const ToeplitzParams P, int nblocks)
         xr[r * xs + (c < kRowPad ? c : ns + c)] = 0.0f;
     }
 
-    // lane's elements of a block: e = tid + NT u, row e / ns, channel e % ns
-    // (ns > 512 and NT <= 1024 < 2 ns: a lane starts at most one row in and
-    // each step wraps at most two rows).  Buffer loads take the block's
-    // extent as their range: elements past the last row read as zero.
-    auto wrap = [ns](int& k, int& r) {
-        if (k >= ns) {
-            k -= ns;
-            ++r;
-        }
-    };
+    // lane's elements of a block: row r, channel k = tid + NT c for the kRC
+    // chunks c of a row (a lane past the row's end reads the next row's
+    // element and does not use it).  One buffer range per block, its extent:
+    // elements past the last row read as zero and their stores are dropped;
+    // the row and chunk offsets are scalar.
+    constexpr int kRC = U / kRows;
+    static_assert(kRC * kRows == U && NT * kRC >= 1536, "a row is kRC chunks of NT channels");
     float v[U];
     auto load_block = [&](int blk, int t0) {
         const float* xb = P.x + static_cast<size_t>(blk) * kRows * ns;  // uniform
         const int nel = min(kRows, P.nrow - blk * kRows) * ns;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float*>(xb), 0, static_cast<unsigned int>(nel) * 4u, 0x00020000);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                const_cast<float*>(xb + NT * u), 0,
-                static_cast<unsigned int>(max(0, nel - NT * u)) * 4u, 0x00020000);
-            v[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                 rs, t0 * 4, 0, 0));
-        }
+        for (int r = 0; r < kRows; ++r)
+#pragma unroll
+            for (int c = 0; c < kRC; ++c)
+                v[r * kRC + c] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_raw_buffer_load_b32(rs, t0 * 4, (r * ns + NT * c) * 4, 0));
     };
     int blk = blockIdx.x;
     if (blk < nblocks) load_block(blk, tid);
@@ -786,15 +782,13 @@ ramp_persist_bf16x3_kernel(const ToeplitzParams P, int nblocks)
         const int t0 = tid + lz;
         const int m = (lane + lz) & 15, g = (lane + lz) >> 4;
         __syncthreads();  // the previous block's rows have left the buffer
-        {
-            int k = t0, r = 0;
-            wrap(k, r);
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (r < kRows) xr[r * xs + kRowPad + k] = v[u] * wl[k];
-                k += NT;
-                wrap(k, r);
-                wrap(k, r);
+        for (int c = 0; c < kRC; ++c) {
+            const int k = t0 + NT * c;
+            if (k < ns) {
+                const float w = wl[k];
+#pragma unroll
+                for (int r = 0; r < kRows; ++r) xr[r * xs + kRowPad + k] = v[r * kRC + c] * w;
             }
         }
         __syncthreads();
@@ -921,19 +915,18 @@ ramp_persist_bf16x3_kernel(const ToeplitzParams P, int nblocks)
             const int t1 = tid + sz;
             float* ob = P.out + static_cast<size_t>(blk) * kRows * ns;  // uniform
             const int nel = min(kRows, P.nrow - blk * kRows) * ns;
-            int k = t1, r = 0;
-            wrap(k, r);
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                ob, 0, static_cast<unsigned int>(nel) * 4u, 0x00020000);
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                    ob + NT * u, 0, static_cast<unsigned int>(max(0, nel - NT * u)) * 4u,
-                    0x00020000);
-                const float y = xr[min(r, kRows - 1) * xs + kRowPad + k];
-                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, y), rs,
-                                                      t1 * 4, 0, 0);
-                k += NT;
-                wrap(k, r);
-                wrap(k, r);
+            for (int c = 0; c < kRC; ++c) {
+                const int k = t1 + NT * c;
+                if (k < ns) {
+#pragma unroll
+                    for (int r = 0; r < kRows; ++r)
+                        __builtin_amdgcn_raw_buffer_store_b32(
+                            __builtin_bit_cast(unsigned int, xr[r * xs + kRowPad + k]), rs, t1 * 4,
+                            (r * ns + NT * c) * 4, 0);
+                }
             }
         }
     }
@@ -1003,7 +996,7 @@ int ramp_toeplitz(const ctr_geom& g, const float* in, float* out, unsigned int n
         const size_t lds = persist_kernel_lds(ns);
         // waves per workgroup, tiles per wave, prefetch floats per lane
         const int nw = 8;
-        auto pk = ns <= 1280 ? ramp_persist_bf16x3_kernel<8, 10, 40>
+        auto pk = ns <= 1280 ? ramp_persist_bf16x3_kernel<8, 10, 48>
                              : ramp_persist_bf16x3_kernel<8, 12, 48>;
         if (lds > 64 * 1024)
             CTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(pk),
