@@ -48,3 +48,35 @@ def test_float_division_identity_is_sensitive():
     a = (rng.random(n) + 0.5).astype(np.float32)
     wrong = (a * (np.float32(1) / b)).astype(np.float32)
     assert (wrong != a / b).any()
+
+
+# The single-view BP (csrc/par_svbp.hip, div_rn) divides by the channel
+# pitch b with y = RN(1/b): q0 = RN(a y), r = a - b q0 (exact in one FMA),
+# q1 = RN(q0 + r y) must be the correctly rounded a / b (Markstein's
+# theorem) for every |a| >= 2^-960.  The FMAs are emulated exactly with
+# rationals; the samples include exact multiples of b and values one ulp
+# away from them, where a wrong correction step would fail first.
+def _fma(x, y, z):
+    from fractions import Fraction
+    return float(Fraction(x) * Fraction(y) + Fraction(z))
+
+
+def _div_rn(a, b, y):
+    q0 = a * y
+    return _fma(_fma(-q0, b, a), y, q0)
+
+
+def test_fp64_division_from_reciprocal_and_two_fmas():
+    rng = np.random.default_rng(20261205)
+    pitches = [1.0, 1.1, 0.65, 1.3, 0.7, 1.0 / 3.0, 0.1, 2.0, 0.9999999999999999]
+    pitches += list((rng.random(20) + 0.5) * np.exp2(rng.integers(-6, 6, 20)))
+    for b in pitches:
+        y = 1.0 / b
+        a = list((rng.random(300) * 2 - 1) * 1000.0)
+        k = rng.integers(-100000, 100000, 300).astype(np.float64)
+        a += list(k * b)  # exact multiples (rounded products)
+        a += list(np.nextafter(k * b, np.inf)) + list(np.nextafter(k * b, -np.inf))
+        for x in a:
+            if abs(x) < 2.0 ** -960:
+                continue
+            assert _div_rn(x, b, y) == x / b, (x, b)
