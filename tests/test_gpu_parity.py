@@ -354,6 +354,29 @@ def test_fp_batches_vs_oracle(F, geo, nbatch):
         assert rel_l2(out[b], O.fp(g, img[b], t)) < TOL32, b
 
 
+# A detector far wider than the image: of the three 200-channel workgroups
+# of a view the outer ones have every ray off the image for whole row
+# chunks (their staging is skipped and the rays only advance), odd image
+# sizes leave a partial last row chunk, and the view counts select one and
+# four views per workgroup; 1, 2 and 5 slices take the one-, two- and
+# four-slice kernels.
+@pytest.mark.parametrize("nbatch", [1, 2, 5])
+@pytest.mark.parametrize("na", [180, 1440])
+def test_fp_wide_detector_vs_oracle(F, na, nbatch):
+    geo = dict(ns=600, na=na, ds=0.5, orbit=math.pi / na, offset_s=0.3, orbit_start=0.05,
+               nx=41, ny=37, dx=1.0, dy=1.1, offset_x=0.4, offset_y=-0.3)
+    g = O.geom(**geo)
+    img = randn(60 + nbatch, (nbatch, g.ny, g.nx))
+    kw = par_kwargs(dict(geo, dso=0, dsd=0))
+    out = host(F.parallel_projection_2d(cuda(img), **kw))
+    t = O.fp_prep(g)
+    for b in range(nbatch):
+        ref = O.fp(g, img[b], t)
+        assert rel_l2(out[b], ref) < TOL32, b
+        # channels whose rays miss the image are exactly zero
+        np.testing.assert_array_equal(out[b][ref == 0], 0.0)
+
+
 # ----------------------------------------------------- filter / weighting
 @pytest.mark.parametrize("typ,name", [(0, "par"), (1, "fan"), (2, "flat")])
 def test_ramp_golden(F, typ, name):
