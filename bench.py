@@ -253,7 +253,8 @@ def bench_raycasting(torch, F, reps, with_cpu=True):
     res = {"workload": "1 x 512^2, parallel 720 views/180deg, 729 ch, ds=dx=1, fov 256, step "
                        "0.125; seeded N(0,1) image", "seed": RC_SEED, "gpu_ms": round(ms, 4),
            "samples": samples, "gsamples_per_s": round(samples / (ms * 1e-3) / 1e9, 2),
-           "fp64_tflops": round(tflops, 3), "fp64_frac": round(tflops / PEAK_FP64_TFLOPS, 4)}
+           "fp64_tflops": round(tflops, 3), "fp64_frac": round(tflops / PEAK_FP64_TFLOPS, 4),
+           "valu_issue": _valu_issue(RC_VALU, ms, "profiles/fp_raycasting_512sq_720v_729ch.md")}
     if with_cpu:
         from oracle import oracle as O
         kind = "ref" if O.have_ref() else "oracle"
@@ -296,6 +297,22 @@ def _dd_roofline(kernel, valu_insts, alg_bytes, ms):
                     "unit": "GB/s",
                     "frac": round(alg_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                     "algorithmic_bytes": alg_bytes}}
+
+
+def _valu_issue(valu_insts, ms, source):
+    """Issue utilisation from a kernel's profiled instruction count: executed
+    VALU wave-instructions x 64 lanes over the measured time."""
+    slots = valu_insts * 64
+    return {"executed_valu_insts": valu_insts, "source": source,
+            "achieved": round(slots / (ms * 1e-3) / 1e12, 3),
+            "peak": round(PEAK_VALU_SLOTS / 1e12, 3), "unit": "Tslot/s (FP32 VALU lane-issues)",
+            "frac": round(slots / (ms * 1e-3) / PEAK_VALU_SLOTS, 4)}
+
+
+# Executed VALU wave-instructions per launch (rocprof SQ_INSTS_VALU) of the
+# ray-casting FP and the helical kernels on their workloads.
+RC_VALU = 1.37e9
+HELI_VALU = {360: 1.39e8, 180: 3.78e8}
 
 
 def bench_fp_dd(torch, F, reps, with_cpu=True):
@@ -683,6 +700,8 @@ def bench_helical(torch, F, reps, with_cpu=True):
                           "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                           "frac": round(gbs / PEAK_HBM_GBS, 4),
                           "compulsory_bytes": bytes_},
+             "valu_issue": _valu_issue(HELI_VALU[mode], ms,
+                                       "profiles/helical_rebin_8640v_32x736.md"),
              "nan_free": bool(torch.isfinite(out).all())}
         if with_cpu:
             from oracle import oracle as O
