@@ -46,8 +46,9 @@ GEO = dict(ns=90, na=100, ds=1.1, orbit=math.pi / 100, offset_s=0.35, orbit_star
            nx=97, ny=61, dx=1.3, dy=1.7, offset_x=-0.4, offset_y=0.6)
 
 
-# na = 37: a view count whose rows are not 16-byte multiples (element-wise
-# table reads and stores instead of 16-byte ones).
+# na = 100: a full 64-view chunk and a partial one; na = 37: one partial
+# chunk, and rows that are not 16-byte multiples (element-wise stores instead
+# of 16-byte ones).  The 97 x 61 image leaves partial 8 x 8 pixel blocks.
 @pytest.mark.parametrize("na", [100, 37])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_svbp_batch_bit_exact_vs_oracle(F, dtype, na):
