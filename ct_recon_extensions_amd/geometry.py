@@ -13,7 +13,7 @@ import ctypes as C
 import hashlib
 import math
 from collections import OrderedDict
-from dataclasses import astuple, dataclass
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -247,7 +247,8 @@ def device_tables(kind: str, p, device, angles=None):
     extra = b""
     if angles is not None:
         extra = hashlib.sha1(np.ascontiguousarray(angles, np.float64).tobytes()).digest()
-    key = (kind, type(p).__name__, astuple(p), str(device), extra)
+    # the parameter structs are frozen dataclasses: hashable by their fields
+    key = (kind, type(p).__name__, p, str(device), extra)
     hit = _CACHE.get(key)
     if hit is not None:
         _CACHE.move_to_end(key)
