@@ -467,10 +467,11 @@ int launch_fp(const FpProblem& P, const T* img, const T* imgT, T* proj, const do
     return check_launch("fp_kernel");
 }
 
-// Views per workgroup, at most kFpViews: 4 measured 0.83 ms on the
-// 8 x 512^2, 720-view scan, 2 0.88 ms (each staged row chunk serves half as
-// many rays) and 8 1.26 ms (234 VGPRs: two waves per SIMD).  Small scans take
-// fewer (fp_views) so that the grid still fills the GPU.
+// Views per workgroup, at most kFpViews: 4 measured 0.78 ms on the
+// 8 x 512^2, 720-view scan, 2 0.89 ms (each staged row chunk serves half as
+// many rays) and, before the chunk pipeline, 8 1.26 ms (two waves per SIMD;
+// 178 VGPRs with it).  Small scans take fewer (fp_views) so that the grid
+// still fills the GPU.
 constexpr int kFpViews = 4;
 constexpr int kFpSlices = 4;
 // Two buffers of 2-row chunks with 512-column spans (32 KB of LDS at 4
