@@ -506,12 +506,15 @@ bool fp_lds_applies(const ctr_geom& g, int views, int rows = kFpRowsDefault,
 }
 
 // Views per workgroup for nb slices per workgroup: the most (up to
-// kFpViews) that leave at least 4 workgroups per CU (1024), else 1.
+// kFpViews) that leave at least 3 workgroups per CU (768 of the 1024 the
+// chip holds at four waves per SIMD), else 1.  On 8 x 512^2 images with
+// 729 channels, 640 to 680 views (960 to 1020 workgroups at 4 views) took
+// 0.73 ms at 4 views against 0.76-0.84 ms at 2, when the bar was 1024.
 int fp_views(const ctr_geom& g, int nbatch, int nb)
 {
     const long long blocks = static_cast<long long>(div_up(g.ns, 256)) * div_up(nbatch, nb);
     for (int v = kFpViews; v > 1; v /= 2)
-        if (blocks * div_up(g.na, v) >= 1024) return v;
+        if (blocks * div_up(g.na, v) >= 768) return v;
     return 1;
 }
 
