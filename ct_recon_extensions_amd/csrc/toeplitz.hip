@@ -19,14 +19,14 @@
 // term, ~2^-16 relative, is dropped).
 //
 // Near field in FP64.  The output is a small difference of large terms: the
-// centre tap h0 and the odd taps |m| <= 15 carry ~97% of the filter's
-// weight and nearly cancel on smooth rows, so FP32 accumulation of them
-// (inside the MFMA) would cost ~2e-5 relative error on CT sinograms.  Those
-// 16 taps are therefore zeroed in the GEMM's tap vector and evaluated in the
-// epilogue in FP64 (FP64 taps times the same FP32 weighted inputs, from an
-// LDS window of the tile's rows), and the three parts are combined in FP64;
-// the matrix cores only carry the far field (~2.5% of the weight), whose
-// FP32 / BF16-split rounding is correspondingly small.
+// centre tap h0 and the first odd taps carry most of the filter's weight and
+// nearly cancel on smooth rows, so FP32 accumulation of them (inside the
+// MFMA) would cost ~2e-5 relative error on CT sinograms.  The odd taps
+// |m| <= 2 kNear - 1 are therefore zeroed in the GEMM's tap vector and
+// evaluated in the epilogue in FP64 (FP64 taps times the same FP32 weighted
+// inputs, from an LDS window of the tile's rows), and the three parts are
+// combined in FP64; the matrix cores only carry the far field, whose FP32 /
+// BF16-split rounding is correspondingly small.
 //
 // Workgroup: 4 waves, a 64-row x 64-output tile of one parity; each wave a
 // 32 x 32 block (2 x 2 MFMA tiles of 16 x 16).  A is staged in LDS in
@@ -48,7 +48,14 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int kTM = 64, kTN = 64, kKC = 64, kAPad = 4;
 // Near-field taps m = 2q + 1, q in [-kNear, kNear - 1] (|m| <= 2 kNear - 1),
 // and the per-row window of opposite-parity inputs they read for a tile.
-constexpr int kNear = 8;
+// With the centre, the taps |m| <= 7 carry most of the filter's weight and
+// nearly cancel on smooth sinograms, so they are summed in FP64; the rest go
+// to the matrix cores.  Worst per-slice relative L2 of the BF16x3 filter on
+// 1440 x 1472 Shepp-Logan-type sinograms, against the FP64 filter, by the
+// largest FP64 tap |m|: 15 -> 3.0e-6, 7 -> 3.4e-6, 5 -> 5.3e-6, 3 -> 9.8e-6,
+// 1 -> 1.5e-5; the 1472-channel filter takes 0.42, 0.39, 0.38, 0.37 and
+// 0.37 ms.
+constexpr int kNear = 4;
 constexpr int kWin = kTN + 2 * kNear;
 
 struct ToeplitzParams {
@@ -456,7 +463,7 @@ __global__ __launch_bounds__(256) void ramp_bf16x3_kernel(const ToeplitzParams P
 constexpr int kRows = 16, kRowPad = 16;
 
 // Row stride (floats): kRowPad + ns + >= 16 zeros (the near field reaches
-// 15 channels past an output), odd, so that the 64 lanes of an A-fragment
+// 2 kNear - 1 channels past an output), odd, so that the 64 lanes of an A-fragment
 // read (16 rows x 4 column groups 16 floats apart) hit 64 different banks.
 __host__ __device__ __forceinline__ int row_stride(int ns) { return (kRowPad + ns + 16) | 1; }
 
